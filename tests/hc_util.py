@@ -1,0 +1,69 @@
+"""ctypes binding of tests/native/libhostcheck.so: the GPU block codec built for the CPU."""
+import ctypes
+import pathlib
+
+import numpy as np
+
+LIB = pathlib.Path(__file__).resolve().parent / "native" / "libhostcheck.so"
+
+
+class HcCfg(ctypes.Structure):
+    _fields_ = [("minbits", ctypes.c_uint32), ("maxbits", ctypes.c_uint32),
+                ("maxprec", ctypes.c_uint32), ("minexp", ctypes.c_int32),
+                ("rounding", ctypes.c_int32), ("tight", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(str(LIB))
+        P = ctypes.POINTER
+        L.hc_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                  P(ctypes.c_int64), P(ctypes.c_int64), P(HcCfg),
+                                  P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
+                                  ctypes.c_uint32, P(ctypes.c_uint32)]
+        L.hc_compress.restype = ctypes.c_uint64
+        L.hc_decompress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                    P(ctypes.c_int64), P(ctypes.c_int64), P(HcCfg),
+                                    P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
+                                    ctypes.c_uint32, P(ctypes.c_uint32)]
+        _lib = L
+    return _lib
+
+
+def _dims(arr):
+    n = list(reversed(arr.shape)) + [1] * (3 - arr.ndim)
+    s = [1, n[0], n[0] * n[1]]
+    return (ctypes.c_int64 * 3)(*n), (ctypes.c_int64 * 3)(*s)
+
+
+def cfg_of(oc):
+    return HcCfg(oc.minbits, oc.maxbits, oc.maxprec, oc.minexp, oc.rounding, oc.tight)
+
+
+def compress(arr, ztype, oc, words, start_bit, fixed_bits, rev):
+    n, s = _dims(arr)
+    nb = 1
+    for v in list(reversed(arr.shape)):
+        nb *= (v + 3) // 4
+    lens = np.zeros(nb, dtype=np.uint32)
+    c = cfg_of(oc)
+    end = lib().hc_compress(ztype, arr.ndim, int(rev), arr.ctypes.data, n, s, ctypes.byref(c),
+                            words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
+                            start_bit, fixed_bits,
+                            lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return end, lens
+
+
+def decompress(shape, dtype, ztype, oc, words, start_bit, fixed_bits, lens, rev):
+    out = np.zeros(shape, dtype=dtype)
+    n, s = _dims(out)
+    c = cfg_of(oc)
+    lp = lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if lens is not None else None
+    lib().hc_decompress(ztype, out.ndim, int(rev), out.ctypes.data, n, s, ctypes.byref(c),
+                        words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
+                        start_bit, fixed_bits, lp)
+    return out

@@ -1,0 +1,105 @@
+"""The GPU block codec's own source (zfp-rs_amd/csrc/zfpx_*.h), built for the CPU,
+against the oracle: stream words and decoded values bit for bit.
+
+This runs without a GPU and checks the exact per-block code and block placement
+(fixed-rate seek, variable-rate length pass + offsets, atomic merge of shared
+words, a start offset inside a word) that the HIP kernels execute.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import hc_util as H
+import oracle_py as O
+from fields import random_walk, smooth, specials
+
+PREFIX_WORD = 0x0123456789abcdef
+PREFIX_PART = 0x5a5a5a5a5a5a5a5a
+
+SHAPES = {1: [(37,), (64,)], 2: [(9, 13), (16, 16)], 3: [(5, 6, 7), (8, 8, 8)]}
+TYPES = [O.INT32, O.INT64, O.FLOAT, O.DOUBLE]
+ROUNDINGS = [(O.ROUND_NEVER, False), (O.ROUND_FIRST, False), (O.ROUND_FIRST, True),
+             (O.ROUND_LAST, False), (O.ROUND_LAST, True)]
+
+
+def is_fixed(cfg, ztype, dims):
+    return (cfg.minbits == cfg.maxbits and
+            O.lib().orc_block_bits(cfg, ztype, dims) == cfg.maxbits)
+
+
+def configs(ztype, dims):
+    out = []
+    for rate in (1, 3, 8, 16, 27):
+        out.append(("rate%g" % rate, O.fixed_rate(rate, ztype, dims)))
+    out.append(("rate8a", O.fixed_rate(8, ztype, dims, align=True)))
+    for p in (5, 16, 33, 64):
+        out.append(("prec%d" % p, O.fixed_precision(p)))
+    for tol in (1e-1, 1e-4, 1e-9):
+        out.append(("acc%g" % tol, O.fixed_accuracy(tol)))
+    out.append(("rev", O.reversible()))
+    out.append(("exp_small", O.config(1, 11, 64, -1074)))
+    out.append(("exp_mixed", O.config(100, 600, 20, -30)))
+    out.append(("exp_revmin", O.config(100, 16658, 64, -1075)))
+    return out
+
+
+def data_for(ztype, shape, kind, seed):
+    dt = O.DTYPES[ztype]
+    if kind == "smooth":
+        return smooth(shape, dt, seed, scale=1e3 if np.dtype(dt).kind == "f" else 1.0)
+    if kind == "walk":
+        return random_walk(shape, dt, seed)
+    return specials(shape, dt, seed)
+
+
+def run_case(arr, ztype, cfg, start_bits):
+    dims = arr.ndim
+    n, err, ow = O.compress(arr, cfg, start_bits=start_bits)
+    assert err == 0
+    nwords = n // 8
+    words = np.zeros(nwords + 4, dtype=np.uint64)
+    words[: start_bits // 64] = PREFIX_WORD
+    if start_bits % 64:
+        words[start_bits // 64] = PREFIX_PART & ((1 << (start_bits % 64)) - 1)
+    fixed = cfg.maxbits if is_fixed(cfg, ztype, dims) else 0
+    rev = cfg.minexp < -1074
+    end, lens = H.compress(arr, ztype, cfg, words, start_bits, fixed, rev)
+    assert (end + 63) // 64 * 8 == n, "stream length"
+    assert np.array_equal(words[:nwords], ow[:nwords]), "stream words"
+    assert not words[nwords:].any()
+    ref, rn, rerr = O.decompress(ow, arr.shape, arr.dtype, cfg, start_bits=start_bits)
+    assert rerr == 0
+    got = H.decompress(arr.shape, arr.dtype, ztype, cfg, words, start_bits, fixed,
+                       None if fixed else lens, rev)
+    isz = arr.dtype.itemsize
+    u = np.uint32 if isz == 4 else np.uint64
+    assert np.array_equal(got.view(u), ref.view(u)), "decoded values"
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+@pytest.mark.parametrize("ztype", TYPES)
+def test_modes_match_oracle(dims, ztype):
+    for shape in SHAPES[dims]:
+        for kind in ("smooth", "specials") if ztype in (O.FLOAT, O.DOUBLE) else ("smooth", "walk", "specials"):
+            arr = data_for(ztype, shape, kind, seed=hash((dims, ztype, shape, kind)) & 0xffff)
+            for name, cfg in configs(ztype, dims):
+                lossy_special = kind == "specials" and cfg.minexp >= -1074
+                for start in (0, 13):
+                    try:
+                        run_case(arr, ztype, cfg, start)
+                    except AssertionError as e:
+                        raise AssertionError(f"{name} {kind} {shape} start={start}: {e}") from None
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+@pytest.mark.parametrize("ztype", TYPES)
+def test_roundings_match_oracle(dims, ztype):
+    shape = SHAPES[dims][0]
+    arr = data_for(ztype, shape, "smooth", seed=7 * dims + ztype)
+    for (rnd, tight), (name, base) in itertools.product(ROUNDINGS, configs(ztype, dims)):
+        cfg = O.config(base.minbits, base.maxbits, base.maxprec, base.minexp, rnd, tight)
+        try:
+            run_case(arr, ztype, cfg, 0)
+        except AssertionError as e:
+            raise AssertionError(f"{name} rounding={rnd} tight={tight}: {e}") from None

@@ -1,0 +1,147 @@
+// zfpx_field.h -- block gather/scatter over a strided 1-D..3-D field, and the
+// per-block encode/decode drivers the kernels call once per thread.
+//
+// Block order is x fastest (field_plan.rs:128-172); a block's origin is
+// 4*(ix*sx + iy*sy + iz*sz) elements from element [0,0,0] and its extent along
+// each axis is min(4, n - 4c) (field_plan.rs:183-200).  Partial blocks are
+// padded as C's pad_block does: position i of a run with n real values reads
+// position i, 0 when i == 3, else n - 1 (strided.rs:716-751).  Scatters store
+// only the in-field values (decode/dim{1,2,3}.rs).
+#pragma once
+
+#include "zfpx_block.h"
+
+namespace zx {
+
+struct FieldDesc {
+  void *data;          // element [0,0,0]; negative strides reach below it
+  int64_t n[3];        // extents (1 for unused axes)
+  int64_t s[3];        // element strides
+  uint32_t nb[3];      // blocks per axis
+  uint64_t nblocks;
+};
+
+ZX_HD uint32_t pad_src(uint32_t i, uint32_t n) { return i < n ? i : (i == 3 ? 0u : n - 1); }
+
+struct BlockPos {
+  int64_t origin;      // element offset of the block origin from data
+  uint32_t len[3];     // values inside the field along each axis
+  bool full;
+};
+
+template <int D>
+ZX_HD BlockPos block_pos(const FieldDesc &f, uint64_t b)
+{
+  BlockPos p;
+  uint32_t c[3] = {0, 0, 0};
+  const uint32_t bb = (uint32_t)b;
+  if constexpr (D == 1) {
+    c[0] = bb;
+  } else if constexpr (D == 2) {
+    c[0] = bb % f.nb[0];
+    c[1] = bb / f.nb[0];
+  } else {
+    c[0] = bb % f.nb[0];
+    const uint32_t t = bb / f.nb[0];
+    c[1] = t % f.nb[1];
+    c[2] = t / f.nb[1];
+  }
+  p.origin = 0;
+  p.full = true;
+  ZX_UNROLL
+  for (int a = 0; a < 3; a++) {
+    if (a < D) {
+      const int64_t o = 4 * (int64_t)c[a];
+      const int64_t rem = f.n[a] - o;
+      p.len[a] = rem < 4 ? (uint32_t)rem : 4u;
+      p.full = p.full && p.len[a] == 4;
+      p.origin += o * f.s[a];
+    } else {
+      p.len[a] = 1;
+    }
+  }
+  return p;
+}
+
+// Load a block's raw value bits, padding partial blocks.
+template <class S, int D>
+ZX_HD void gather(const FieldDesc &f, const BlockPos &p,
+                  typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE])
+{
+  typedef typename Ints<ST<S>::P>::U U;
+  const U *base = (const U *)f.data + p.origin;
+  if (p.full) {
+    ZX_UNROLL
+    for (int i = 0; i < Blk<D>::SIZE; i++) {
+      int64_t off = (int64_t)(i & 3) * f.s[0];
+      if (D >= 2) off += (int64_t)((i >> 2) & 3) * f.s[1];
+      if (D >= 3) off += (int64_t)((i >> 4) & 3) * f.s[2];
+      v[i] = base[off];
+    }
+  } else {
+    ZX_UNROLL
+    for (int i = 0; i < Blk<D>::SIZE; i++) {
+      int64_t off = (int64_t)pad_src((uint32_t)(i & 3), p.len[0]) * f.s[0];
+      if (D >= 2) off += (int64_t)pad_src((uint32_t)((i >> 2) & 3), p.len[1]) * f.s[1];
+      if (D >= 3) off += (int64_t)pad_src((uint32_t)((i >> 4) & 3), p.len[2]) * f.s[2];
+      v[i] = base[off];
+    }
+  }
+}
+
+// Store a block's raw value bits; only values inside the field.
+template <class S, int D>
+ZX_HD void scatter(const FieldDesc &f, const BlockPos &p,
+                   const typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE])
+{
+  typedef typename Ints<ST<S>::P>::U U;
+  U *base = (U *)f.data + p.origin;
+  ZX_UNROLL
+  for (int i = 0; i < Blk<D>::SIZE; i++) {
+    const uint32_t x = i & 3, y = (i >> 2) & 3, z = (i >> 4) & 3;
+    bool in = p.full || (x < p.len[0] && (D < 2 || y < p.len[1]) && (D < 3 || z < p.len[2]));
+    if (in) {
+      int64_t off = (int64_t)x * f.s[0];
+      if (D >= 2) off += (int64_t)y * f.s[1];
+      if (D >= 3) off += (int64_t)z * f.s[2];
+      base[off] = v[i];
+    }
+  }
+}
+
+template <class S, int D, bool REV, class Sink>
+ZX_HD uint32_t encode_values(Sink &w, const typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE],
+                             const Cfg &c)
+{
+  if constexpr (REV) return encode_reversible<S, D>(w, v, c);
+  else return encode_lossy<S, D>(w, v, c);
+}
+
+template <class S, int D, bool REV, class Src>
+ZX_HD void decode_values(Src &r, typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE], const Cfg &c)
+{
+  if constexpr (REV) decode_reversible<S, D>(r, v, c);
+  else decode_lossy<S, D>(r, v, c);
+}
+
+// Encode block b of the field through `w`; returns its bit count.
+template <class S, int D, bool REV, class Sink>
+ZX_HD uint32_t encode_field_block(const FieldDesc &f, uint64_t b, Sink &w, const Cfg &c)
+{
+  typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+  const BlockPos p = block_pos<D>(f, b);
+  gather<S, D>(f, p, v);
+  return encode_values<S, D, REV>(w, v, c);
+}
+
+// Decode block b of the field from `r`.
+template <class S, int D, bool REV, class Src>
+ZX_HD void decode_field_block(const FieldDesc &f, uint64_t b, Src &r, const Cfg &c)
+{
+  typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+  decode_values<S, D, REV>(r, v, c);
+  const BlockPos p = block_pos<D>(f, b);
+  scatter<S, D>(f, p, v);
+}
+
+}  // namespace zx
