@@ -1,0 +1,54 @@
+/*
+ * zfp_hip.h -- extensions of the MI355X ZFP codec beyond zfp-rs's C ABI.
+ *
+ * None of these has a counterpart in zfp-rs-ffi's C header; each replaces a
+ * capability the reference offers through its Rust API:
+ *  - rounding per stream: ZfpConfig::with_rounding (src/config.rs:115-174),
+ *    which zfp-rs-ffi fixes at build time (zfp-rs-ffi/src/stream.rs:12-16);
+ *  - the execution stream: ZfpExecution (src/execution.rs:12-93) picks a
+ *    thread pool; here a HIP stream;
+ *  - the offset index: the per-block bit lengths of a variable-rate stream,
+ *    which the reference rediscovers serially on every decode
+ *    (src/decompress/pipeline.rs:139-229) and this library records at
+ *    compress time beside the unchanged stream.
+ */
+#ifndef ZFP_HIP_EXT_H
+#define ZFP_HIP_EXT_H
+
+#include "zfp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Rounding of later compress/decompress calls through this stream:
+ * ZFP_ROUND_NEVER (default), ZFP_ROUND_FIRST or ZFP_ROUND_LAST, with
+ * ZFP_WITH_TIGHT_ERROR when tight_error is true.  Returns zfp_false for an
+ * unknown mode.  Not recorded in the stream: decode with the same setting. */
+zfp_bool zfp_stream_set_rounding(zfp_stream *stream, int rounding, zfp_bool tight_error);
+int zfp_stream_rounding(const zfp_stream *stream, zfp_bool *tight_error);
+
+/* HIP stream (hipStream_t) the codec's kernels run on; NULL = the default
+ * stream.  zfp_compress/zfp_decompress still return only when done. */
+zfp_bool zfp_stream_set_hip_stream(zfp_stream *stream, void *hip_stream);
+
+/* Offset index of the last variable-rate compression through a bit stream:
+ * one uint32 bit length per block, in block order, for the blocks written
+ * from bit `start_bit`.  zfp_decompress uses a matching index (same start,
+ * block count and parameters) and otherwise walks the stream on the GPU.
+ * export: returns the block count (copies at most `capacity` lengths when
+ * `lengths` is non-null); 0 when the stream holds no index.
+ * import: attach an index read from elsewhere; returns zfp_false on failure. */
+size_t zfp_stream_index_export(const bitstream *stream, uint32_t *lengths, size_t capacity,
+                               uint64 *start_bit);
+zfp_bool zfp_stream_index_import(bitstream *stream, const uint32_t *lengths, size_t count,
+                                 uint64 start_bit);
+
+/* Device the library runs on (the current HIP device when first used). */
+int zfp_hip_device(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZFP_HIP_EXT_H */

@@ -1,0 +1,1531 @@
+// zfp_capi.cpp -- the C ABI (include/zfp.h, include/zfp_hip.h) over the HIP
+// field codec.
+//
+// Mirrors zfp-rs-ffi (zfp-rs-ffi/src/{stream,field,bitstream}.rs): the same
+// entry points, argument meaning and error behaviour (0 / null / zfp_false on
+// failure, never a crash for bad parameters).  The bit-stream cursor keeps
+// the reference's exact semantics (src/bitstream/core.rs), so header I/O and
+// user bit I/O interleave with GPU compression exactly as with zfp-rs.
+//
+// zfp_compress / zfp_decompress run every block on the GPU; there is no CPU
+// codec in this library.  Host-resident fields and streams are staged through
+// device memory; device-resident ones are coded in place.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "../../include/zfp.h"
+#include "../../include/zfp_hip.h"
+#include "zfpx_api.h"
+
+extern "C" {
+const size_t stream_word_bits = 64;
+const uint32_t zfp_codec_version = ZFP_CODEC;
+const uint32_t zfp_library_version = 10001;  // 1.0.1 as major*10000 + minor*100 + patch
+}
+
+// ===================================================================== memory
+
+namespace {
+
+bool is_device_ptr(const void *p)
+{
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+// Grow-only device buffers shared by the codec calls of this process.
+struct Workspace {
+  std::mutex mu;
+  void *field = nullptr; size_t field_cap = 0;
+  void *words = nullptr; size_t words_cap = 0;
+  void *lens = nullptr; size_t lens_cap = 0;
+  void *offs = nullptr; size_t offs_cap = 0;
+  void *tiles = nullptr; size_t tiles_cap = 0;
+  void *one = nullptr;  // one pinned host word for reading back a scan total
+  int device = -1;
+
+  static bool grow(void *&p, size_t &cap, size_t need)
+  {
+    if (need <= cap) return true;
+    const size_t n = std::max(need, cap + cap / 2);
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, n) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return false;
+    }
+    cap = n;
+    return true;
+  }
+};
+
+Workspace &ws()
+{
+  static Workspace w;
+  return w;
+}
+
+}  // namespace
+
+// ===================================================================== bit stream
+
+// A bit stream over 64-bit words in host or device memory, with the cursor of
+// src/bitstream/core.rs:7-18.
+struct bitstream {
+  uint64_t *words;
+  size_t nwords;
+  bool owned;
+  bool device;
+  size_t word_pos;
+  uint64_t buffer;
+  uint32_t bits;
+  bool overflowed;
+  bool overread;
+  size_t seek_word;
+  // offset index of the last variable-rate compression (device memory)
+  uint32_t *index;
+  size_t index_n;
+  size_t index_cap;
+  uint64_t index_start;
+  uint64_t index_key;
+
+  uint64_t get(size_t i) const
+  {
+    if (i >= nwords) return 0;
+    if (!device) return words[i];
+    uint64_t v = 0;
+    (void)hipMemcpy(&v, words + i, 8, hipMemcpyDeviceToHost);
+    return v;
+  }
+  void set(size_t i, uint64_t v)
+  {
+    if (!device) words[i] = v;
+    else (void)hipMemcpy(words + i, &v, 8, hipMemcpyHostToDevice);
+  }
+  // read_word_raw / write_word_raw (core.rs:151-170)
+  uint64_t load()
+  {
+    uint64_t w = get(word_pos);
+    word_pos++;
+    return w;
+  }
+  void store(uint64_t v)
+  {
+    if (word_pos < nwords) set(word_pos, v);
+    else overflowed = true;
+    word_pos++;
+  }
+  // read_bits_impl (core.rs:192-229)
+  uint64_t read_bits(uint32_t n)
+  {
+    if (n > 64) n = 64;
+    uint64_t value = buffer;
+    if (bits < n) {
+      uint64_t w = load();
+      buffer = w;
+      value += w << bits;
+      bits = bits + 64 - n;
+      if (bits == 0) {
+        buffer = 0;
+      } else {
+        buffer >>= 64 - bits;
+        if (n < 64) value &= (2ull << (n - 1)) - 1;
+      }
+    } else {
+      bits -= n;
+      buffer = n < 64 ? buffer >> n : 0;
+      if (n < 64) value &= (1ull << n) - 1;
+    }
+    return value;
+  }
+  uint32_t read_bit()
+  {
+    if (bits == 0) {
+      buffer = load();
+      bits = 64;
+    }
+    bits--;
+    uint32_t b = (uint32_t)(buffer & 1);
+    buffer >>= 1;
+    return b;
+  }
+  // write_bits_impl (core.rs:296-326)
+  uint64_t write_bits(uint64_t value, uint32_t n)
+  {
+    if (n > 64) n = 64;
+    uint32_t old = bits;
+    buffer += value << bits;
+    bits += n;
+    if (bits >= 64) {
+      uint64_t full = buffer;
+      bits -= 64;
+      store(full);
+      buffer = (value >> 1) >> (63 - old);
+    }
+    if (bits < 64) buffer &= (1ull << bits) - 1;
+    return n < 64 ? value >> n : 0;
+  }
+  void write_bit(uint32_t bit)
+  {
+    buffer += (uint64_t)bit << bits;
+    bits++;
+    if (bits == 64) {
+      store(buffer);
+      buffer = 0;
+      bits = 0;
+    }
+  }
+  bool loaded_since_seek() const { return word_pos > std::max(nwords, seek_word); }
+  // seek_read_impl (core.rs:265-284)
+  void rseek(uint64_t off)
+  {
+    uint32_t n = (uint32_t)(off % 64);
+    overread = overread || loaded_since_seek();
+    word_pos = (size_t)(off / 64);
+    seek_word = word_pos;
+    if (n) {
+      buffer = load() >> n;
+      bits = 64 - n;
+    } else {
+      buffer = 0;
+      bits = 0;
+    }
+  }
+  // seek_write_impl (core.rs:354-375)
+  void wseek(uint64_t off)
+  {
+    uint32_t n = (uint32_t)(off % 64);
+    word_pos = (size_t)(off / 64);
+    overflowed = false;
+    if (n) {
+      buffer = word_pos < nwords ? get(word_pos) & ((1ull << n) - 1) : 0;
+      bits = n;
+    } else {
+      buffer = 0;
+      bits = 0;
+    }
+  }
+  uint64_t wtell() const { return (uint64_t)word_pos * 64 + bits; }
+  uint64_t rtell() const { return (uint64_t)word_pos * 64 - bits; }
+  void skip(uint64_t n) { rseek(rtell() + n); }
+  // pad_impl (core.rs:460-478)
+  void pad(uint64_t n)
+  {
+    uint64_t b = (uint64_t)bits + n;
+    while (b >= 64) {
+      if (word_pos >= nwords) {
+        word_pos += (size_t)(b / 64);
+        buffer = 0;
+        overflowed = true;
+        b %= 64;
+        break;
+      }
+      store(buffer);
+      buffer = 0;
+      b -= 64;
+    }
+    bits = (uint32_t)b;
+  }
+  uint32_t align()
+  {
+    uint32_t b = bits;
+    if (b) skip(b);
+    return b;
+  }
+  uint32_t flush()
+  {
+    uint32_t p = (64 - bits) % 64;
+    if (p) pad(p);
+    return p;
+  }
+  void rewind()
+  {
+    word_pos = 0;
+    buffer = 0;
+    bits = 0;
+    overflowed = false;
+    overread = false;
+    seek_word = 0;
+  }
+  bool overread_now() const { return overread || loaded_since_seek(); }
+  void reset_overread()
+  {
+    overread = false;
+    seek_word = bits > 0 ? (word_pos ? word_pos - 1 : 0) : word_pos;
+  }
+};
+
+namespace {
+
+bitstream *new_stream(uint64_t *words, size_t nwords, bool owned, bool device)
+{
+  bitstream *s = (bitstream *)calloc(1, sizeof(bitstream));
+  if (!s) return nullptr;
+  s->words = words;
+  s->nwords = nwords;
+  s->owned = owned;
+  s->device = device;
+  return s;
+}
+
+void drop_index(bitstream *s)
+{
+  if (s->index) (void)hipFree(s->index);
+  s->index = nullptr;
+  s->index_n = s->index_cap = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// stream_open (zfp-rs-ffi/src/bitstream.rs:168-198): a null buffer allocates,
+// an aligned one is borrowed, an unaligned host buffer is copied.
+bitstream *stream_open(void *buffer, size_t bytes)
+{
+  if (!buffer) {
+    size_t n = (bytes + 7) / 8;
+    uint64_t *w = (uint64_t *)calloc(n ? n : 1, 8);
+    if (!w) return nullptr;
+    return new_stream(w, n, true, false);
+  }
+  const bool dev = is_device_ptr(buffer);
+  if (((uintptr_t)buffer % 8) != 0) {
+    if (dev) return nullptr;
+    size_t n = (bytes + 7) / 8;
+    uint64_t *w = (uint64_t *)calloc(n ? n : 1, 8);
+    if (!w) return nullptr;
+    memcpy(w, buffer, bytes);
+    return new_stream(w, n, true, false);
+  }
+  return new_stream((uint64_t *)buffer, bytes / 8, false, dev);
+}
+
+void stream_close(bitstream *s)
+{
+  if (!s) return;
+  drop_index(s);
+  if (s->owned) free(s->words);
+  free(s);
+}
+
+// deep copy of the committed words (C stream_clone)
+bitstream *stream_clone(const bitstream *s)
+{
+  if (!s) return nullptr;
+  size_t n = std::min(s->word_pos, s->nwords);
+  uint64_t *w = (uint64_t *)calloc(n ? n : 1, 8);
+  if (!w) return nullptr;
+  if (n) {
+    if (s->device) (void)hipMemcpy(w, s->words, n * 8, hipMemcpyDeviceToHost);
+    else memcpy(w, s->words, n * 8);
+  }
+  return new_stream(w, n, true, false);
+}
+
+bitstream_count stream_alignment(void) { return 8; }
+void *stream_data(const bitstream *s) { return s ? (void *)s->words : nullptr; }
+size_t stream_size(const bitstream *s) { return s ? s->word_pos * 8 : 0; }
+size_t stream_capacity(const bitstream *s) { return s ? s->nwords * 8 : 0; }
+size_t stream_stride_block(const bitstream *) { return 8; }
+ptrdiff_t stream_stride_delta(const bitstream *) { return -8; }
+uint stream_read_bit(bitstream *s) { return s ? s->read_bit() : 0; }
+uint stream_write_bit(bitstream *s, uint bit)
+{
+  if (!s) return 0;
+  s->write_bit(bit & 1u);
+  return bit;
+}
+uint64 stream_read_bits(bitstream *s, bitstream_count n)
+{
+  return s ? s->read_bits(n > 64 ? 64u : (uint32_t)n) : 0;
+}
+uint64 stream_write_bits(bitstream *s, uint64 value, bitstream_count n)
+{
+  return s ? s->write_bits(value, n > 64 ? 64u : (uint32_t)n) : 0;
+}
+bitstream_offset stream_rtell(const bitstream *s) { return s ? s->rtell() : 0; }
+bitstream_offset stream_wtell(const bitstream *s) { return s ? s->wtell() : 0; }
+void stream_rewind(bitstream *s)
+{
+  if (s) s->rewind();
+}
+void stream_rseek(bitstream *s, bitstream_offset o)
+{
+  if (s) s->rseek(o);
+}
+void stream_wseek(bitstream *s, bitstream_offset o)
+{
+  if (s) s->wseek(o);
+}
+void stream_skip(bitstream *s, bitstream_size n)
+{
+  if (s) s->skip(n);
+}
+void stream_pad(bitstream *s, bitstream_size n)
+{
+  if (s) s->pad(n);
+}
+bitstream_count stream_align(bitstream *s) { return s ? s->align() : 0; }
+bitstream_count stream_flush(bitstream *s) { return s ? s->flush() : 0; }
+
+// copy_impl (core.rs:490-522)
+void stream_copy(bitstream *dst, bitstream *src, bitstream_size n)
+{
+  if (!dst || !src) return;
+  uint64_t rem = n;
+  while (rem > 64) {
+    bool src_done = src->word_pos >= src->nwords && src->buffer == 0;
+    bool dst_done = dst->word_pos >= dst->nwords;
+    if (src_done && dst_done) {
+      uint64_t skipped = (rem - 1) / 64;
+      src->word_pos += (size_t)skipped;
+      dst->word_pos += (size_t)skipped;
+      dst->buffer = 0;
+      dst->overflowed = true;
+      rem -= skipped * 64;
+      break;
+    }
+    dst->write_bits(src->read_bits(64), 64);
+    rem -= 64;
+  }
+  if (rem > 0) dst->write_bits(src->read_bits((uint32_t)rem), (uint32_t)rem);
+}
+
+}  // extern "C"
+
+// ===================================================================== config
+
+namespace {
+
+const uint32_t kMinBits = ZFP_MIN_BITS, kMaxBits = ZFP_MAX_BITS, kMaxPrec = ZFP_MAX_PREC;
+const int32_t kMinExp = ZFP_MIN_EXP;
+
+uint32_t header_bits(zfp_type t) { return t == zfp_type_float ? 9 : t == zfp_type_double ? 12 : 0; }
+uint32_t type_prec(zfp_type t) { return (t == zfp_type_int32 || t == zfp_type_float) ? 32 : 64; }
+size_t type_size(zfp_type t)
+{
+  return (t == zfp_type_int32 || t == zfp_type_float) ? 4 : (t == zfp_type_int64 || t == zfp_type_double) ? 8 : 0;
+}
+bool valid_type(zfp_type t) { return t >= zfp_type_int32 && t <= zfp_type_double; }
+
+bool valid_params(uint32_t minbits, uint32_t maxbits, uint32_t maxprec)
+{
+  return minbits <= maxbits && maxprec > 0 && maxprec <= 64;
+}
+
+void set_params(zfp_stream *z, uint32_t a, uint32_t b, uint32_t c, int32_t d)
+{
+  z->minbits = a;
+  z->maxbits = b;
+  z->maxprec = c;
+  z->minexp = d;
+}
+
+// ZfpConfig::mode (config.rs:469-527)
+zfp_mode mode_of(uint32_t minb, uint32_t maxb, uint32_t maxp, int32_t mine)
+{
+  if (!valid_params(minb, maxb, maxp)) return zfp_mode_null;
+  if (minb == kMinBits && maxb == kMaxBits && maxp == kMaxPrec && mine == kMinExp) return zfp_mode_expert;
+  if (minb == maxb && maxb >= 1 && maxb <= kMaxBits && maxp >= kMaxPrec && mine == kMinExp)
+    return zfp_mode_fixed_rate;
+  if (minb <= kMinBits && maxb >= kMaxBits && maxp >= 1 && mine == kMinExp) return zfp_mode_fixed_precision;
+  if (minb <= kMinBits && maxb >= kMaxBits && maxp >= kMaxPrec && mine >= kMinExp)
+    return zfp_mode_fixed_accuracy;
+  if (minb <= kMinBits && maxb >= kMaxBits && maxp >= kMaxPrec && mine < kMinExp) return zfp_mode_reversible;
+  return zfp_mode_expert;
+}
+zfp_mode mode_of(const zfp_stream *z) { return mode_of(z->minbits, z->maxbits, z->maxprec, z->minexp); }
+
+// encode_expert_mode (config.rs:178-198)
+uint64_t expert_mode_bits(uint32_t minb, uint32_t maxb, uint32_t maxp, int32_t mine)
+{
+  uint64_t a = std::min(std::max(minb, 1u), 0x8000u) - 1;
+  uint64_t b = std::min(std::max(maxb, 1u), 0x8000u) - 1;
+  uint64_t c = std::min(std::max(maxp, 1u), 0x80u) - 1;
+  int64_t e = (int64_t)mine + 16495;
+  if (e < 0) e = 0;
+  if (e > 0x7fff) e = 0x7fff;
+  uint64_t m = 0;
+  m = (m << 15) | (uint64_t)e;
+  m = (m << 7) | c;
+  m = (m << 15) | b;
+  m = (m << 15) | a;
+  m = (m << 12) | 0xfffu;
+  return m;
+}
+
+// mode_bits (config.rs:570-600)
+uint64_t mode_bits(uint32_t minb, uint32_t maxb, uint32_t maxp, int32_t mine)
+{
+  switch (mode_of(minb, maxb, maxp, mine)) {
+  case zfp_mode_fixed_rate:
+    if (maxb <= 2048) return maxb - 1;
+    break;
+  case zfp_mode_fixed_precision:
+    if (maxp <= 128) return (uint64_t)(maxp - 1) + 2048;
+    break;
+  case zfp_mode_reversible: return 2048 + 128;
+  case zfp_mode_fixed_accuracy:
+    if (mine <= 843) return (uint64_t)(mine - kMinExp) + (2048 + 128 + 1);
+    break;
+  default: break;
+  }
+  return expert_mode_bits(minb, maxb, maxp, mine);
+}
+
+// from_mode_bits (config.rs:215-247); false when the parameters are invalid
+bool params_from_mode(uint64_t m, uint32_t &a, uint32_t &b, uint32_t &c, int32_t &d)
+{
+  if (m <= ZFP_MODE_SHORT_MAX) {
+    if (m < 2048) { a = b = (uint32_t)m + 1; c = kMaxPrec; d = kMinExp; }
+    else if (m < 2048 + 128) { a = kMinBits; b = kMaxBits; c = (uint32_t)m + 1 - 2048; d = kMinExp; }
+    else if (m == 2048 + 128) { a = kMinBits; b = kMaxBits; c = kMaxPrec; d = kMinExp - 1; }
+    else { a = kMinBits; b = kMaxBits; c = kMaxPrec; d = (int32_t)m + kMinExp - (2048 + 128 + 1); }
+  } else {
+    uint64_t x = m >> 12;
+    a = (uint32_t)(x & 0x7fff) + 1; x >>= 15;
+    b = (uint32_t)(x & 0x7fff) + 1; x >>= 15;
+    c = (uint32_t)(x & 0x7f) + 1; x >>= 7;
+    d = (int32_t)(x & 0x7fff) - 16495;
+  }
+  return valid_params(a, b, c);
+}
+
+// rate_bits (config.rs:85-102); 0 on failure
+bool rate_bits(double rate, zfp_type t, uint32_t dims, bool align, uint32_t &out)
+{
+  double values = (double)(1u << (2 * dims));
+  double r = floor(values * rate + 0.5);
+  if (!(r >= 0.0 && r <= 4294967295.0)) return false;
+  uint32_t b = (uint32_t)r;
+  b = std::max(b, header_bits(t));
+  if (align) {
+    uint64_t x = ((uint64_t)b + 63) / 64 * 64;
+    if (x > 0xffffffffull) return false;
+    b = (uint32_t)x;
+  }
+  out = b;
+  return true;
+}
+
+// block_bits (config.rs:670-686)
+uint32_t block_bits(const zx::Cfg &c, zfp_type t, int dims)
+{
+  uint32_t values = 1u << (2 * dims);
+  uint32_t header;
+  if (c.minexp < kMinExp)
+    header = t == zfp_type_int32 ? 5 : t == zfp_type_int64 ? 6 : t == zfp_type_float ? 15 : 19;
+  else
+    header = header_bits(t);
+  uint64_t most = (uint64_t)header + values - 1 + (uint64_t)values * std::min(c.maxprec, type_prec(t));
+  most = std::min<uint64_t>(most, c.maxbits);
+  most = std::max<uint64_t>(most, header);
+  most = std::max<uint64_t>(most, c.minbits);
+  return (uint32_t)most;
+}
+
+// Library-owned execution parameters; they begin as zfp_exec_params_omp does.
+struct ExecParams {
+  uint threads;
+  uint chunk_size;
+  int32_t rounding;  // zx::ROUND_*
+  int32_t tight;
+  hipStream_t hip;
+};
+
+ExecParams *params(zfp_stream *z, bool create)
+{
+  if (!z->exec.params && create) z->exec.params = calloc(1, sizeof(ExecParams));
+  return (ExecParams *)z->exec.params;
+}
+const ExecParams *params(const zfp_stream *z) { return (const ExecParams *)z->exec.params; }
+
+zx::Cfg cfg_of(const zfp_stream *z)
+{
+  const ExecParams *p = params(z);
+  return zx::Cfg{z->minbits, z->maxbits, z->maxprec, z->minexp, p ? p->rounding : 0,
+                 p ? p->tight : 0};
+}
+
+hipStream_t hip_stream_of(const zfp_stream *z)
+{
+  const ExecParams *p = params(z);
+  return p ? p->hip : nullptr;
+}
+
+// ===================================================================== fields
+
+int dimensionality(const zfp_field *f)
+{
+  return f->ny == 0 ? 1 : f->nz == 0 ? 2 : f->nw == 0 ? 3 : 4;
+}
+
+struct Plan {
+  zfp_type type;
+  int dims;
+  int64_t n[4];
+  int64_t s[4];
+  int64_t imin, imax;  // element span relative to element [0,0,0,0]
+  size_t esz;
+  uint64_t nblocks;
+  uint32_t nb[3];
+};
+
+// checked_index_span + effective strides (field.rs:580-647), block grid
+// (field_plan.rs:41-77); false where the reference rejects the field.
+bool make_plan(const zfp_field *f, Plan &p)
+{
+  if (!f || !valid_type(f->type)) return false;
+  p.type = f->type;
+  p.esz = type_size(f->type);
+  p.dims = dimensionality(f);
+  const size_t dims[4] = {f->nx, f->ny, f->nz, f->nw};
+  const ptrdiff_t strides[4] = {f->sx, f->sy, f->sz, f->sw};
+  int64_t below = 1;
+  bool below_ok = true;
+  p.imin = p.imax = 0;
+  p.nblocks = 1;
+  for (int a = 0; a < 4; a++) {
+    p.n[a] = 1;
+    p.s[a] = 0;
+  }
+  for (int a = 0; a < p.dims; a++) {
+    const int64_t d = (int64_t)dims[a];
+    if (d < 0) return false;
+    int64_t st = strides[a];
+    if (st == 0) {
+      if (!below_ok) return false;
+      st = below;
+    }
+    p.n[a] = d;
+    p.s[a] = st;
+    if (d > 0) {
+      __int128 ext = (__int128)st * (d - 1);
+      if (ext > INT64_MAX / 2 || ext < INT64_MIN / 2) return false;
+      if (ext < 0) p.imin += (int64_t)ext;
+      else p.imax += (int64_t)ext;
+    }
+    __int128 nbl = (__int128)below * d;
+    if (nbl > INT64_MAX) below_ok = false;
+    else below = (int64_t)nbl;
+    p.nblocks *= (uint64_t)((d + 3) / 4);
+  }
+  for (int a = 0; a < 3; a++) p.nb[a] = (uint32_t)((p.n[a] + 3) / 4);
+  if (p.nblocks > 0xffffffffull) return false;
+  return true;
+}
+
+bool contiguous(const Plan &p)
+{
+  int64_t nat = 1;
+  for (int a = 0; a < p.dims; a++) {
+    if (p.s[a] != nat) return false;
+    nat *= p.n[a];
+  }
+  return true;
+}
+
+}  // namespace
+
+// ===================================================================== codec
+
+namespace {
+
+hipError_t launch_encode(zfp_type t, int dims, bool rev, const zx::EncodeArgs &a, hipStream_t s)
+{
+  switch (t) {
+  case zfp_type_int32: return zx_launch_encode_i32(dims, rev, a, s);
+  case zfp_type_int64: return zx_launch_encode_i64(dims, rev, a, s);
+  case zfp_type_float: return zx_launch_encode_f32(dims, rev, a, s);
+  case zfp_type_double: return zx_launch_encode_f64(dims, rev, a, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_decode(zfp_type t, int dims, bool rev, const zx::DecodeArgs &a, uint32_t *walk,
+                         hipStream_t s)
+{
+  switch (t) {
+  case zfp_type_int32: return zx_launch_decode_i32(dims, rev, a, walk, s);
+  case zfp_type_int64: return zx_launch_decode_i64(dims, rev, a, walk, s);
+  case zfp_type_float: return zx_launch_decode_f32(dims, rev, a, walk, s);
+  case zfp_type_double: return zx_launch_decode_f64(dims, rev, a, walk, s);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+uint64_t cfg_key(const zx::Cfg &c, const Plan &p)
+{
+  uint64_t h = 1469598103934665603ull;
+  const uint64_t v[] = {c.minbits, c.maxbits, c.maxprec, (uint64_t)(uint32_t)c.minexp,
+                        (uint64_t)c.rounding, (uint64_t)c.tight, (uint64_t)p.type,
+                        (uint64_t)p.dims, p.nblocks};
+  for (uint64_t x : v) h = (h ^ x) * 1099511628211ull;
+  return h;
+}
+
+zx::FieldDesc field_desc(const Plan &p, void *origin)
+{
+  zx::FieldDesc f;
+  f.data = origin;
+  for (int a = 0; a < 3; a++) {
+    f.n[a] = p.n[a];
+    f.s[a] = p.s[a];
+    f.nb[a] = p.nb[a];
+  }
+  f.nblocks = p.nblocks;
+  return f;
+}
+
+bool vec_ok(const Plan &p, const void *origin)
+{
+  if (p.s[0] != 1) return false;
+  if (p.dims >= 2 && (p.s[1] % 4) != 0) return false;
+  if (p.dims >= 3 && (p.s[2] % 4) != 0) return false;
+  return ((uintptr_t)origin % 16) == 0;
+}
+
+bool fixed_layout(const zx::Cfg &c, const Plan &p)
+{
+  return c.minbits == c.maxbits && block_bits(c, p.type, p.dims) == c.maxbits && c.maxbits > 0;
+}
+
+#define ZCHECK(x)                     \
+  do {                                \
+    if ((x) != hipSuccess) {          \
+      (void)hipGetLastError();        \
+      return false;                   \
+    }                                 \
+  } while (0)
+
+// Device pointer to element [0,0,0] of the field: the caller's own, or the
+// span staged into the workspace (copied in when `copy_in`).
+bool stage_field(Workspace &W, const zfp_field *f, const Plan &p, bool copy_in, hipStream_t st,
+                 void *&dorigin, bool &staged)
+{
+  staged = !is_device_ptr(f->data);
+  if (!staged) {
+    dorigin = f->data;
+    return true;
+  }
+  const size_t span = (size_t)(p.imax - p.imin + 1) * p.esz;
+  if (!Workspace::grow(W.field, W.field_cap, span)) return false;
+  const char *lo = (const char *)f->data + p.imin * (int64_t)p.esz;
+  if (copy_in) ZCHECK(hipMemcpyAsync(W.field, lo, span, hipMemcpyHostToDevice, st));
+  dorigin = (char *)W.field - p.imin * (int64_t)p.esz;
+  return true;
+}
+
+// Encode every block of the field at the cursor of `s`; leaves the cursor
+// where compress + finish would (compress.rs:22-53).  Returns false on a
+// device failure (nothing consistent was written).
+bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st)
+{
+  Workspace &W = ws();
+  std::lock_guard<std::mutex> lock(W.mu);
+  const uint64_t start = s->wtell();
+  const uint64_t prefix = s->bits ? s->buffer : 0;
+  const uint64_t w0 = start / 64;
+  const bool rev = c.minexp < kMinExp;
+  void *dorigin = nullptr;
+  bool staged = false;
+  if (!stage_field(W, f, p, true, st, dorigin, staged)) return false;
+  zx::EncodeArgs a;
+  a.field = field_desc(p, dorigin);
+  a.cfg = c;
+  a.vec = vec_ok(p, dorigin);
+  a.offsets = nullptr;
+  a.lengths = nullptr;
+  a.start = start - 64 * w0;
+  const bool fixed = fixed_layout(c, p);
+  uint64_t end_rel;
+  uint64_t *offs = nullptr;
+  if (fixed) {
+    a.fixed = c.maxbits;
+    end_rel = a.start + p.nblocks * (uint64_t)c.maxbits;
+  } else {
+    a.fixed = 0;
+    if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
+    if (!Workspace::grow(W.offs, W.offs_cap, (p.nblocks + 1) * 8)) return false;
+    if (!Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(p.nblocks) * 8)) return false;
+    if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
+    a.lengths = (uint32_t *)W.lens;
+    a.words = nullptr;
+    a.nwords = 0;
+    ZCHECK(launch_encode(p.type, p.dims, rev, a, st));
+    offs = (uint64_t *)W.offs;
+    ZCHECK(zx_launch_scan((uint32_t *)W.lens, p.nblocks, a.start, offs, (uint64_t *)W.tiles, st));
+    ZCHECK(hipMemcpyAsync(W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
+    ZCHECK(hipStreamSynchronize(st));
+    end_rel = *(uint64_t *)W.one;
+    a.lengths = nullptr;
+    a.offsets = offs;
+  }
+  const uint64_t w1 = (64 * w0 + end_rel + 63) / 64;  // words through the flush
+  const uint64_t cap_words = s->nwords > w0 ? s->nwords - w0 : 0;
+  const uint64_t live = std::min<uint64_t>(w1, s->nwords) > w0 ? std::min<uint64_t>(w1, s->nwords) - w0 : 0;
+  uint64_t *dwords;
+  if (s->device) {
+    dwords = s->words + w0;
+  } else {
+    if (!Workspace::grow(W.words, W.words_cap, std::max<uint64_t>(live, 1) * 8)) return false;
+    dwords = (uint64_t *)W.words;
+  }
+  a.words = dwords;
+  a.nwords = std::min(cap_words, live);
+  const bool aligned = a.start == 0 && fixed && (c.maxbits % 64) == 0;
+  if (!aligned && live) ZCHECK(zx_launch_fill(dwords, live, prefix, st));
+  if (live) ZCHECK(launch_encode(p.type, p.dims, rev, a, st));
+  if (!s->device && live) {
+    ZCHECK(hipMemcpyAsync(s->words + w0, dwords, live * 8, hipMemcpyDeviceToHost, st));
+  }
+  if (!fixed) {
+    // keep the offset index beside the stream
+    if (s->index_cap < p.nblocks) {
+      drop_index(s);
+      if (hipMalloc((void **)&s->index, p.nblocks * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        s->index = nullptr;
+      } else {
+        s->index_cap = p.nblocks;
+      }
+    }
+    if (s->index) {
+      ZCHECK(hipMemcpyAsync(s->index, W.lens, p.nblocks * 4, hipMemcpyDeviceToDevice, st));
+      s->index_n = p.nblocks;
+      s->index_start = start;
+      s->index_key = cfg_key(c, p);
+    }
+  }
+  ZCHECK(hipStreamSynchronize(st));
+  // the cursor after the last block, then flush (core.rs:536-542)
+  if (w1 > s->nwords) s->overflowed = true;
+  s->word_pos = (size_t)w1;
+  s->bits = 0;
+  s->buffer = 0;
+  return true;
+}
+
+// Decode every block of the field from the cursor of `s`; leaves the cursor
+// and overread state where decompress + finish would (decompress.rs:27-74).
+bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st,
+                    uint64_t &end_out)
+{
+  Workspace &W = ws();
+  std::lock_guard<std::mutex> lock(W.mu);
+  const uint64_t start = s->rtell();
+  const uint64_t w0 = start / 64;
+  const bool rev = c.minexp < kMinExp;
+  s->reset_overread();
+  const size_t seek_word0 = s->seek_word;
+  void *dorigin = nullptr;
+  bool staged = false;
+  if (!stage_field(W, f, p, !contiguous(p), st, dorigin, staged)) return false;
+  zx::DecodeArgs a;
+  a.field = field_desc(p, dorigin);
+  a.cfg = c;
+  a.vec = vec_ok(p, dorigin);
+  a.start = start - 64 * w0;
+  a.offsets = nullptr;
+  if (!Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(p.nblocks) * 8 + 8)) return false;
+  // the last tiles word collects the highest word any block loaded
+  uint64_t *maxword = (uint64_t *)W.tiles + zx_scan_tiles(p.nblocks);
+  a.maxword = maxword;
+  const bool fixed = fixed_layout(c, p);
+  // device words from w0 on
+  const uint64_t avail = s->nwords > w0 ? s->nwords - w0 : 0;
+  uint64_t need = avail;
+  if (fixed) {
+    const uint64_t end_rel = a.start + p.nblocks * (uint64_t)c.maxbits;
+    need = std::min<uint64_t>(avail, (end_rel + 63) / 64);
+  }
+  const uint64_t *dwords;
+  if (s->device) {
+    dwords = s->words + w0;
+  } else {
+    if (!Workspace::grow(W.words, W.words_cap, std::max<uint64_t>(need, 1) * 8)) return false;
+    if (need) ZCHECK(hipMemcpyAsync(W.words, s->words + w0, need * 8, hipMemcpyHostToDevice, st));
+    dwords = (const uint64_t *)W.words;
+  }
+  a.words = dwords;
+  a.nwords = need;
+  uint64_t end_rel;
+  if (fixed) {
+    a.fixed = c.maxbits;
+    end_rel = a.start + p.nblocks * (uint64_t)c.maxbits;
+  } else {
+    a.fixed = 0;
+    if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
+    if (!Workspace::grow(W.offs, W.offs_cap, (p.nblocks + 1) * 8)) return false;
+    if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
+    const uint32_t *lens;
+    if (s->index && s->index_n == p.nblocks && s->index_start == start &&
+        (s->index_key == 0 || s->index_key == cfg_key(c, p))) {
+      lens = s->index;
+    } else {
+      // no index: walk the stream on the GPU to find block lengths
+      ZCHECK(launch_decode(p.type, p.dims, rev, a, (uint32_t *)W.lens, st));
+      lens = (const uint32_t *)W.lens;
+    }
+    uint64_t *offs = (uint64_t *)W.offs;
+    ZCHECK(zx_launch_scan(lens, p.nblocks, a.start, offs, (uint64_t *)W.tiles, st));
+    ZCHECK(hipMemcpyAsync(W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
+    a.offsets = offs;
+  }
+  if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
+  ZCHECK(zx_launch_fill(maxword, 1, 0, st));
+  ZCHECK(launch_decode(p.type, p.dims, rev, a, nullptr, st));
+  ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 1, maxword, 8, hipMemcpyDeviceToHost, st));
+  if (staged) {
+    const size_t span = (size_t)(p.imax - p.imin + 1) * p.esz;
+    char *lo = (char *)f->data + p.imin * (int64_t)p.esz;
+    ZCHECK(hipMemcpyAsync(lo, W.field, span, hipMemcpyDeviceToHost, st));
+  }
+  ZCHECK(hipStreamSynchronize(st));
+  if (!fixed) end_rel = *(uint64_t *)W.one;
+  const uint64_t end = 64 * w0 + end_rel;
+  end_out = end;
+  // park + align: the cursor at the word boundary after the last block.  A
+  // load past the end (not a skip over missing padding) is a truncation.
+  const uint64_t w1 = (end + 63) / 64;
+  const uint64_t loaded_end = w0 + ((uint64_t *)W.one)[1];  // one past the highest loaded word
+  const bool over = loaded_end > std::max<uint64_t>(s->nwords, seek_word0);
+  s->word_pos = (size_t)w1;
+  s->bits = 0;
+  s->buffer = 0;
+  s->seek_word = s->word_pos;
+  s->overread = over;
+  return true;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+size_t zfp_type_size(zfp_type t) { return type_size(t); }
+
+zfp_config zfp_config_none(void)
+{
+  zfp_config c;
+  memset(&c, 0, sizeof(c));
+  c.mode = zfp_mode_null;
+  return c;
+}
+zfp_config zfp_config_rate(double rate, zfp_bool align)
+{
+  zfp_config c = zfp_config_none();
+  c.mode = zfp_mode_fixed_rate;
+  c.arg.rate = (align == zfp_false && rate >= 0.0) ? -rate : rate;  // negative: unaligned
+  return c;
+}
+zfp_config zfp_config_precision(uint32_t precision)
+{
+  zfp_config c = zfp_config_none();
+  c.mode = zfp_mode_fixed_precision;
+  c.arg.precision = precision;
+  return c;
+}
+zfp_config zfp_config_accuracy(double tolerance)
+{
+  zfp_config c = zfp_config_none();
+  c.mode = zfp_mode_fixed_accuracy;
+  c.arg.tolerance = tolerance;
+  return c;
+}
+zfp_config zfp_config_reversible(void)
+{
+  zfp_config c = zfp_config_none();
+  c.mode = zfp_mode_reversible;
+  return c;
+}
+zfp_config zfp_config_expert(uint32_t minbits, uint32_t maxbits, uint32_t maxprec, int32_t minexp)
+{
+  zfp_config c = zfp_config_none();
+  c.mode = zfp_mode_expert;
+  c.arg.expert.minbits = minbits;
+  c.arg.expert.maxbits = maxbits;
+  c.arg.expert.maxprec = maxprec;
+  c.arg.expert.minexp = minexp;
+  return c;
+}
+
+zfp_stream *zfp_stream_open(bitstream *bs)
+{
+  zfp_stream *z = (zfp_stream *)calloc(1, sizeof(zfp_stream));
+  if (!z) return nullptr;
+  set_params(z, kMinBits, kMaxBits, kMaxPrec, kMinExp);
+  z->stream = bs;
+  z->exec.policy = zfp_exec_serial;
+  z->exec.params = nullptr;
+  return z;
+}
+
+void zfp_stream_close(zfp_stream *z)
+{
+  if (!z) return;
+  free(z->exec.params);
+  free(z);
+}
+
+bitstream *zfp_stream_bit_stream(const zfp_stream *z) { return z ? z->stream : nullptr; }
+zfp_mode zfp_stream_compression_mode(const zfp_stream *z) { return z ? mode_of(z) : zfp_mode_null; }
+
+double zfp_stream_rate(const zfp_stream *z, uint dims)
+{
+  if (!z || dims < 1 || dims > 4 || mode_of(z) != zfp_mode_fixed_rate) return 0.0;
+  return (double)z->maxbits / (double)(1u << (2 * dims));
+}
+uint zfp_stream_precision(const zfp_stream *z)
+{
+  return z && mode_of(z) == zfp_mode_fixed_precision ? z->maxprec : 0;
+}
+double zfp_stream_accuracy(const zfp_stream *z)
+{
+  return z && mode_of(z) == zfp_mode_fixed_accuracy ? ldexp(1.0, z->minexp) : 0.0;
+}
+uint64 zfp_stream_mode(const zfp_stream *z)
+{
+  return z ? mode_bits(z->minbits, z->maxbits, z->maxprec, z->minexp) : 0;
+}
+void zfp_stream_params(const zfp_stream *z, uint *minbits, uint *maxbits, uint *maxprec, int32_t *minexp)
+{
+  if (!z || !minbits || !maxbits || !maxprec || !minexp) return;
+  *minbits = z->minbits;
+  *maxbits = z->maxbits;
+  *maxprec = z->maxprec;
+  *minexp = z->minexp;
+}
+size_t zfp_stream_compressed_size(const zfp_stream *z) { return z && z->stream ? z->stream->word_pos * 8 : 0; }
+
+// zfp_block_maximum_size (zfp-rs-ffi/src/stream.rs:178-206)
+size_t zfp_block_maximum_size(zfp_type ty, uint dims, zfp_bool reversible)
+{
+  static const size_t sizes[2][4][4] = {
+      {{131, 527, 2111, 8447}, {259, 1039, 4159, 16639}, {140, 536, 2120, 8456}, {271, 1051, 4171, 16651}},
+      {{136, 532, 2116, 8452}, {265, 1045, 4165, 16645}, {146, 542, 2126, 8462}, {278, 1058, 4178, 16658}},
+  };
+  if (!valid_type(ty) || dims < 1 || dims > 4) return 0;
+  return sizes[reversible != zfp_false][ty - 1][dims - 1];
+}
+
+// maximum_size (config.rs:642-658)
+size_t zfp_stream_maximum_size(const zfp_stream *z, const zfp_field *f)
+{
+  if (!z || !f || !valid_type(f->type)) return 0;
+  size_t d[4] = {f->nx, f->ny, f->nz, f->nw};
+  for (int i = 0; i < 4; i++)
+    if (d[i] == 0) {
+      for (int j = i; j < 4; j++) d[j] = 0;
+      break;
+    }
+  if (d[0] == 0) return 0;
+  int dims = d[1] == 0 ? 1 : d[2] == 0 ? 2 : d[3] == 0 ? 3 : 4;
+  unsigned __int128 blocks = 1;
+  for (int i = 0; i < dims; i++) blocks *= (d[i] + 3) / 4;
+  zx::Cfg c = cfg_of(z);
+  unsigned __int128 bits = blocks * block_bits(c, f->type, dims) + 148;
+  bits = (bits + 63) / 64 * 64;
+  if (bits / 8 > (unsigned __int128)SIZE_MAX) return 0;
+  return (size_t)(bits / 8);
+}
+
+// field metadata (field.rs:678-735)
+static bool field_metadata(const zfp_field *f, uint64_t &meta)
+{
+  const size_t d[4] = {f->nx, f->ny, f->nz, f->nw};
+  if (d[0] == 0) return false;
+  bool zero = false;
+  for (int i = 1; i < 4; i++) {
+    if (d[i] == 0) zero = true;
+    else if (zero) return false;
+  }
+  int dims = dimensionality(f);
+  int width = 48 / dims;
+  uint64_t maxv = (1ull << width) - 1;
+  uint64_t m = 0;
+  int shift = 0;
+  for (int i = 0; i < dims; i++) {
+    uint64_t e = (uint64_t)d[i] - 1;
+    if (e > maxv) return false;
+    m |= e << shift;
+    shift += width;
+  }
+  m = (m << 2) | (uint64_t)(dims - 1);
+  m = (m << 2) | (uint64_t)(f->type - 1);
+  meta = m;
+  return true;
+}
+
+static bool metadata_to_field(uint64_t meta, zfp_field *f)
+{
+  if (meta >> 52) return false;
+  zfp_type t = (zfp_type)((meta & 3) + 1);
+  uint64_t m = meta >> 2;
+  int dims = (int)(m & 3) + 1;
+  m >>= 2;
+  int width = 48 / dims;
+  uint64_t mask = (1ull << width) - 1;
+  size_t d[4] = {0, 0, 0, 0};
+  for (int i = 0; i < dims; i++) {
+    d[i] = (size_t)((m & mask) + 1);
+    m >>= width;
+  }
+  f->type = t;
+  f->nx = d[0]; f->ny = d[1]; f->nz = d[2]; f->nw = d[3];
+  f->sx = f->sy = f->sz = f->sw = 0;
+  return true;
+}
+
+// zfp_write_header (zfp-rs-ffi/src/stream.rs:230-276)
+size_t zfp_write_header(zfp_stream *z, const zfp_field *f, uint mask)
+{
+  if (!z || !f || !z->stream || !valid_type(f->type)) return 0;
+  Plan p;
+  if (!make_plan(f, p)) return 0;
+  bitstream *s = z->stream;
+  size_t bits = 0;
+  if (mask & ZFP_HEADER_MAGIC) {
+    s->write_bits('z', 8);
+    s->write_bits('f', 8);
+    s->write_bits('p', 8);
+    s->write_bits(ZFP_CODEC, 8);
+    bits += ZFP_MAGIC_BITS;
+  }
+  if (mask & ZFP_HEADER_META) {
+    uint64_t meta;
+    if (!field_metadata(f, meta)) return 0;
+    s->write_bits(meta, ZFP_META_BITS);
+    bits += ZFP_META_BITS;
+  }
+  if (mask & ZFP_HEADER_MODE) {
+    uint64_t m = mode_bits(z->minbits, z->maxbits, z->maxprec, z->minexp);
+    uint32_t size = m > ZFP_MODE_SHORT_MAX ? ZFP_MODE_LONG_BITS : ZFP_MODE_SHORT_BITS;
+    s->write_bits(m, size);
+    bits += size;
+  }
+  return bits;
+}
+
+// zfp_read_header (header.rs:132-173)
+size_t zfp_read_header(zfp_stream *z, zfp_field *f, uint mask)
+{
+  if (!z || !f || !z->stream) return 0;
+  bitstream *s = z->stream;
+  size_t bits = 0;
+  if (mask & ZFP_HEADER_MAGIC) {
+    if (s->read_bits(8) != 'z' || s->read_bits(8) != 'f' || s->read_bits(8) != 'p' ||
+        s->read_bits(8) != ZFP_CODEC)
+      return 0;
+    bits += ZFP_MAGIC_BITS;
+  }
+  zfp_field tmp = *f;
+  bool have_meta = false;
+  if (mask & ZFP_HEADER_META) {
+    uint64_t meta = s->read_bits(ZFP_META_BITS);
+    if (!metadata_to_field(meta, &tmp)) return 0;
+    have_meta = true;
+    bits += ZFP_META_BITS;
+  }
+  uint32_t a = 0, b = 0, c = 0;
+  int32_t d = 0;
+  bool have_mode = false;
+  if (mask & ZFP_HEADER_MODE) {
+    uint64_t m = s->read_bits(ZFP_MODE_SHORT_BITS);
+    bits += ZFP_MODE_SHORT_BITS;
+    if (m > ZFP_MODE_SHORT_MAX) {
+      m |= s->read_bits(ZFP_MODE_LONG_BITS - ZFP_MODE_SHORT_BITS) << ZFP_MODE_SHORT_BITS;
+      bits += ZFP_MODE_LONG_BITS - ZFP_MODE_SHORT_BITS;
+    }
+    if (!params_from_mode(m, a, b, c, d)) return 0;
+    have_mode = true;
+  }
+  if (have_meta) {
+    f->type = tmp.type;
+    f->nx = tmp.nx; f->ny = tmp.ny; f->nz = tmp.nz; f->nw = tmp.nw;
+    f->sx = f->sy = f->sz = f->sw = 0;
+  }
+  if (have_mode) set_params(z, a, b, c, d);
+  return bits;
+}
+
+// zfp_compress (zfp-rs-ffi/src/stream.rs:321-342): bytes of stream after the
+// flush, or 0 on any failure.
+size_t zfp_compress(zfp_stream *z, const zfp_field *f)
+{
+  if (!z || !f || !z->stream || !f->data) return 0;
+  Plan p;
+  if (!make_plan(f, p)) return 0;
+  if (p.dims > 3) return 0;  // 4-D blocks are not built for the GPU
+  if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return 0;
+  bitstream *s = z->stream;
+  const zx::Cfg c = cfg_of(z);
+  if (p.nblocks > 0) {
+    if (!gpu_compress(s, f, p, c, hip_stream_of(z))) return 0;
+  } else {
+    s->flush();
+  }
+  if (s->overflowed) return 0;
+  return s->word_pos * 8;
+}
+
+// zfp_decompress (zfp-rs-ffi/src/stream.rs:345-367): bytes read through the
+// aligned end of the last block, or 0 on any failure (truncation included).
+size_t zfp_decompress(zfp_stream *z, zfp_field *f)
+{
+  if (!z || !f || !z->stream || !f->data || !valid_type(f->type)) return 0;
+  Plan p;
+  if (!make_plan(f, p)) return 0;
+  if (p.dims > 3) return 0;
+  if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return 0;
+  bitstream *s = z->stream;
+  const zx::Cfg c = cfg_of(z);
+  if (p.nblocks > 0) {
+    uint64_t end;
+    if (!gpu_decompress(s, f, p, c, hip_stream_of(z), end)) return 0;
+  } else {
+    s->reset_overread();
+    s->align();
+  }
+  if (s->overread_now()) return 0;
+  return s->word_pos * 8;
+}
+
+void zfp_stream_rewind(zfp_stream *z)
+{
+  if (z && z->stream) z->stream->rewind();
+}
+void zfp_stream_set_bit_stream(zfp_stream *z, bitstream *bs)
+{
+  if (z) z->stream = bs;
+}
+void zfp_stream_set_reversible(zfp_stream *z)
+{
+  if (z) set_params(z, kMinBits, kMaxBits, kMaxPrec, kMinExp - 1);
+}
+
+// zfp_stream_set_rate (zfp-rs-ffi/src/stream.rs:393-423)
+double zfp_stream_set_rate(zfp_stream *z, double rate, zfp_type ty, uint dims, zfp_bool align)
+{
+  if (!z || dims < 1 || dims > 4 || !valid_type(ty)) return 0.0;
+  uint32_t bits;
+  if (!rate_bits(rate, ty, dims, align == zfp_true, bits)) return 0.0;
+  set_params(z, bits, bits, kMaxPrec, kMinExp);
+  return (double)bits / (double)(1u << (2 * dims));
+}
+
+uint zfp_stream_set_precision(zfp_stream *z, uint precision)
+{
+  if (!z) return 0;
+  uint32_t p = precision > 0 ? std::min(precision, kMaxPrec) : kMaxPrec;
+  set_params(z, kMinBits, kMaxBits, p, kMinExp);
+  return p;
+}
+
+double zfp_stream_set_accuracy(zfp_stream *z, double tolerance)
+{
+  if (!z) return 0.0;
+  int32_t emin = kMinExp;
+  if (tolerance > 0.0) {
+    int e;
+    frexp(tolerance, &e);
+    if (isinf(tolerance)) e = 0;  // frexp's exponent of infinity is taken as zero
+    emin = e - 1;
+  }
+  set_params(z, kMinBits, kMaxBits, kMaxPrec, emin);
+  return tolerance > 0.0 ? ldexp(1.0, emin) : 0.0;
+}
+
+zfp_mode zfp_stream_set_mode(zfp_stream *z, uint64 mode)
+{
+  if (!z) return zfp_mode_null;
+  uint32_t a, b, c;
+  int32_t d;
+  if (!params_from_mode(mode, a, b, c, d)) return zfp_mode_null;
+  set_params(z, a, b, c, d);
+  return mode_of(z);
+}
+
+zfp_bool zfp_stream_set_params(zfp_stream *z, uint minbits, uint maxbits, uint maxprec, int32_t minexp)
+{
+  if (!z || !valid_params(minbits, maxbits, maxprec)) return zfp_false;
+  set_params(z, minbits, maxbits, maxprec, minexp);
+  return zfp_true;
+}
+
+zfp_exec_policy zfp_stream_execution(const zfp_stream *z) { return z ? z->exec.policy : zfp_exec_serial; }
+uint zfp_stream_omp_threads(const zfp_stream *z) { return z && params(z) ? params(z)->threads : 0; }
+uint zfp_stream_omp_chunk_size(const zfp_stream *z) { return z && params(z) ? params(z)->chunk_size : 0; }
+
+zfp_bool zfp_stream_set_execution(zfp_stream *z, zfp_exec_policy policy)
+{
+  if (!z) return zfp_false;
+  switch (policy) {
+  case zfp_exec_serial:
+    if (ExecParams *p = params(z, false)) p->threads = p->chunk_size = 0;
+    z->exec.policy = policy;
+    return zfp_true;
+  case zfp_exec_omp:
+  case zfp_exec_hip:
+    z->exec.policy = policy;
+    return zfp_true;
+  default:
+    return zfp_false;
+  }
+}
+
+zfp_bool zfp_stream_set_omp_threads(zfp_stream *z, uint threads)
+{
+  if (!z) return zfp_false;
+  ExecParams *p = params(z, true);
+  if (!p) return zfp_false;
+  p->threads = threads;
+  p->chunk_size = 0;
+  z->exec.policy = zfp_exec_omp;
+  return zfp_true;
+}
+
+zfp_bool zfp_stream_set_omp_chunk_size(zfp_stream *z, uint chunk_size)
+{
+  if (!z) return zfp_false;
+  ExecParams *p = params(z, true);
+  if (!p) return zfp_false;
+  p->chunk_size = chunk_size;
+  z->exec.policy = zfp_exec_omp;
+  return zfp_true;
+}
+
+size_t zfp_stream_flush(zfp_stream *z) { return z && z->stream ? z->stream->flush() : 0; }
+size_t zfp_stream_align(zfp_stream *z) { return z && z->stream ? z->stream->align() : 0; }
+
+// ---- fields (zfp-rs-ffi/src/field.rs)
+
+static zfp_field *alloc_field(void *data, zfp_type t, size_t nx, size_t ny, size_t nz, size_t nw)
+{
+  zfp_field *f = (zfp_field *)calloc(1, sizeof(zfp_field));
+  if (!f) return nullptr;
+  f->type = t;
+  f->nx = nx; f->ny = ny; f->nz = nz; f->nw = nw;
+  f->data = data;
+  return f;
+}
+zfp_field *zfp_field_alloc(void) { return alloc_field(nullptr, zfp_type_none, 0, 0, 0, 0); }
+zfp_field *zfp_field_1d(void *d, zfp_type t, size_t nx) { return alloc_field(d, t, nx, 0, 0, 0); }
+zfp_field *zfp_field_2d(void *d, zfp_type t, size_t nx, size_t ny) { return alloc_field(d, t, nx, ny, 0, 0); }
+zfp_field *zfp_field_3d(void *d, zfp_type t, size_t nx, size_t ny, size_t nz)
+{
+  return alloc_field(d, t, nx, ny, nz, 0);
+}
+zfp_field *zfp_field_4d(void *d, zfp_type t, size_t nx, size_t ny, size_t nz, size_t nw)
+{
+  return alloc_field(d, t, nx, ny, nz, nw);
+}
+void zfp_field_free(zfp_field *f) { free(f); }
+void *zfp_field_pointer(const zfp_field *f) { return f ? f->data : nullptr; }
+void *zfp_field_begin(const zfp_field *f)
+{
+  Plan p;
+  if (!f || !f->data || !make_plan(f, p)) return nullptr;
+  return (char *)f->data + p.imin * (int64_t)p.esz;
+}
+zfp_type zfp_field_type(const zfp_field *f) { return f ? f->type : zfp_type_none; }
+uint zfp_field_precision(const zfp_field *f)
+{
+  return f && valid_type(f->type) ? type_prec(f->type) : 0;
+}
+uint zfp_field_dimensionality(const zfp_field *f)
+{
+  Plan p;
+  return f && make_plan(f, p) ? (uint)p.dims : 0;
+}
+size_t zfp_field_size(const zfp_field *f, size_t *dims)
+{
+  if (!f) return 0;
+  if (dims) {
+    dims[0] = f->nx; dims[1] = f->ny; dims[2] = f->nz; dims[3] = f->nw;
+  }
+  Plan p;
+  if (!make_plan(f, p)) return 0;
+  size_t n = 1;
+  for (int a = 0; a < p.dims; a++) n *= (size_t)p.n[a];
+  return n;
+}
+size_t zfp_field_size_bytes(const zfp_field *f)
+{
+  Plan p;
+  if (!f || !make_plan(f, p)) return 0;
+  return (size_t)(p.imax - p.imin + 1) * p.esz;
+}
+size_t zfp_field_blocks(const zfp_field *f)
+{
+  Plan p;
+  return f && make_plan(f, p) ? (size_t)p.nblocks : 0;
+}
+zfp_bool zfp_field_is_contiguous(const zfp_field *f)
+{
+  Plan p;
+  if (!f || !make_plan(f, p)) return zfp_false;
+  return contiguous(p) ? zfp_true : zfp_false;
+}
+zfp_bool zfp_field_stride(const zfp_field *f, ptrdiff_t *strides)
+{
+  if (!f) return zfp_false;
+  if (strides) {
+    strides[0] = f->sx; strides[1] = f->sy; strides[2] = f->sz; strides[3] = f->sw;
+  }
+  return zfp_field_is_contiguous(f) ? zfp_false : zfp_true;
+}
+uint64 zfp_field_metadata(const zfp_field *f)
+{
+  if (!f) return 0;
+  uint64_t m;
+  if (!valid_type(f->type) || !field_metadata(f, m)) return UINT64_MAX;
+  return m;
+}
+void zfp_field_set_pointer(zfp_field *f, void *ptr)
+{
+  if (f) f->data = ptr;
+}
+zfp_type zfp_field_set_type(zfp_field *f, zfp_type t)
+{
+  if (!f) return zfp_type_int32;
+  if (!valid_type(t)) return zfp_type_none;
+  f->type = t;
+  return t;
+}
+void zfp_field_set_size_1d(zfp_field *f, size_t nx)
+{
+  if (f) { f->nx = nx; f->ny = f->nz = f->nw = 0; }
+}
+void zfp_field_set_size_2d(zfp_field *f, size_t nx, size_t ny)
+{
+  if (f) { f->nx = nx; f->ny = ny; f->nz = f->nw = 0; }
+}
+void zfp_field_set_size_3d(zfp_field *f, size_t nx, size_t ny, size_t nz)
+{
+  if (f) { f->nx = nx; f->ny = ny; f->nz = nz; f->nw = 0; }
+}
+void zfp_field_set_size_4d(zfp_field *f, size_t nx, size_t ny, size_t nz, size_t nw)
+{
+  if (f) { f->nx = nx; f->ny = ny; f->nz = nz; f->nw = nw; }
+}
+void zfp_field_set_stride_1d(zfp_field *f, ptrdiff_t sx)
+{
+  if (f) { f->sx = sx; f->sy = f->sz = f->sw = 0; }
+}
+void zfp_field_set_stride_2d(zfp_field *f, ptrdiff_t sx, ptrdiff_t sy)
+{
+  if (f) { f->sx = sx; f->sy = sy; f->sz = f->sw = 0; }
+}
+void zfp_field_set_stride_3d(zfp_field *f, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz)
+{
+  if (f) { f->sx = sx; f->sy = sy; f->sz = sz; f->sw = 0; }
+}
+void zfp_field_set_stride_4d(zfp_field *f, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz, ptrdiff_t sw)
+{
+  if (f) { f->sx = sx; f->sy = sy; f->sz = sz; f->sw = sw; }
+}
+zfp_bool zfp_field_set_metadata(zfp_field *f, uint64 meta)
+{
+  if (!f) return zfp_false;
+  return metadata_to_field(meta, f) ? zfp_true : zfp_false;
+}
+
+// ---- extensions (include/zfp_hip.h)
+
+zfp_bool zfp_stream_set_rounding(zfp_stream *z, int rounding, zfp_bool tight)
+{
+  if (!z) return zfp_false;
+  int r;
+  if (rounding == ZFP_ROUND_NEVER) r = zx::ROUND_NEVER;
+  else if (rounding == ZFP_ROUND_FIRST) r = zx::ROUND_FIRST;
+  else if (rounding == ZFP_ROUND_LAST) r = zx::ROUND_LAST;
+  else return zfp_false;
+  ExecParams *p = params(z, true);
+  if (!p) return zfp_false;
+  p->rounding = r;
+  p->tight = tight ? 1 : 0;
+  return zfp_true;
+}
+
+int zfp_stream_rounding(const zfp_stream *z, zfp_bool *tight)
+{
+  const ExecParams *p = z ? params(z) : nullptr;
+  if (tight) *tight = p && p->tight ? zfp_true : zfp_false;
+  if (!p) return ZFP_ROUND_NEVER;
+  return p->rounding == zx::ROUND_FIRST ? ZFP_ROUND_FIRST
+         : p->rounding == zx::ROUND_LAST ? ZFP_ROUND_LAST : ZFP_ROUND_NEVER;
+}
+
+zfp_bool zfp_stream_set_hip_stream(zfp_stream *z, void *hs)
+{
+  if (!z) return zfp_false;
+  ExecParams *p = params(z, true);
+  if (!p) return zfp_false;
+  p->hip = (hipStream_t)hs;
+  return zfp_true;
+}
+
+size_t zfp_stream_index_export(const bitstream *s, uint32_t *lengths, size_t capacity, uint64 *start)
+{
+  if (!s || !s->index || !s->index_n) return 0;
+  if (start) *start = s->index_start;
+  if (lengths) {
+    size_t n = std::min(capacity, s->index_n);
+    if (hipMemcpy(lengths, s->index, n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+  }
+  return s->index_n;
+}
+
+zfp_bool zfp_stream_index_import(bitstream *s, const uint32_t *lengths, size_t count, uint64 start)
+{
+  if (!s || !lengths || !count) return zfp_false;
+  if (s->index_cap < count) {
+    drop_index(s);
+    if (hipMalloc((void **)&s->index, count * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      s->index = nullptr;
+      return zfp_false;
+    }
+    s->index_cap = count;
+  }
+  if (hipMemcpy(s->index, lengths, count * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    return zfp_false;
+  }
+  s->index_n = count;
+  s->index_start = start;
+  s->index_key = 0;  // an imported index matches on start and block count
+  return zfp_true;
+}
+
+int zfp_hip_device(void)
+{
+  int d = -1;
+  if (hipGetDevice(&d) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  return d;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// zfpx_scan.hip -- exclusive scan of per-block bit lengths into stream offsets
+// (variable-rate placement), and the fill that zeroes a stream's word range.
+//
+// Three passes: per-tile sums, one workgroup scanning the tile sums, then each
+// tile rescans its lengths with the tile's base added.  A tile is 256 threads
+// x 8 lengths.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "zfpx_api.h"
+
+namespace {
+
+constexpr int kT = 256;
+constexpr int kItems = 8;
+constexpr uint64_t kTile = (uint64_t)kT * kItems;
+
+__global__ __launch_bounds__(kT) void k_tile_sums(const uint32_t *len, uint64_t n, uint64_t *tiles)
+{
+  typedef hipcub::BlockReduce<uint64_t, kT> Reduce;
+  __shared__ typename Reduce::TempStorage tmp;
+  const uint64_t base = (uint64_t)blockIdx.x * kTile;
+  uint64_t s = 0;
+  for (int i = 0; i < kItems; i++) {
+    const uint64_t j = base + (uint64_t)i * kT + threadIdx.x;
+    if (j < n) s += len[j];
+  }
+  const uint64_t t = Reduce(tmp).Sum(s);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kT) void k_scan_tiles(uint64_t *tiles, uint64_t ntiles, uint64_t start)
+{
+  typedef hipcub::BlockScan<uint64_t, kT> Scan;
+  __shared__ typename Scan::TempStorage tmp;
+  __shared__ uint64_t carry;
+  if (threadIdx.x == 0) carry = start;
+  __syncthreads();
+  for (uint64_t b = 0; b < ntiles; b += kT) {
+    const uint64_t j = b + threadIdx.x;
+    const uint64_t v = j < ntiles ? tiles[j] : 0;
+    uint64_t ex, total;
+    Scan(tmp).ExclusiveSum(v, ex, total);
+    const uint64_t c = carry;
+    if (j < ntiles) tiles[j] = c + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry = c + total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tiles[ntiles] = carry;
+}
+
+__global__ __launch_bounds__(kT) void k_apply(const uint32_t *len, uint64_t n, const uint64_t *tiles,
+                                              uint64_t *off)
+{
+  typedef hipcub::BlockScan<uint64_t, kT> Scan;
+  __shared__ typename Scan::TempStorage tmp;
+  // thread t owns lengths base + t*kItems .. + kItems (blocked arrangement)
+  const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+  uint64_t v[kItems];
+  uint64_t s = 0;
+  for (int i = 0; i < kItems; i++) {
+    v[i] = base + i < n ? len[base + i] : 0;
+    s += v[i];
+  }
+  uint64_t ex;
+  Scan(tmp).ExclusiveSum(s, ex);
+  uint64_t o = tiles[blockIdx.x] + ex;
+  for (int i = 0; i < kItems; i++) {
+    if (base + i < n) off[base + i] = o;
+    o += v[i];
+  }
+  if (base < n && base + kItems >= n) off[n] = o;  // the thread holding the last length
+}
+
+__global__ void k_fill(uint64_t *w, uint64_t n, uint64_t first)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = i == 0 ? first : 0ull;
+}
+
+}  // namespace
+
+uint64_t zx_scan_tiles(uint64_t n) { return (n + kTile - 1) / kTile + 1; }
+
+hipError_t zx_launch_scan(const uint32_t *lengths, uint64_t n, uint64_t start, uint64_t *offsets,
+                          uint64_t *tiles, hipStream_t s)
+{
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(1), 0, s, offsets, (uint64_t)1, start);
+    return hipGetLastError();
+  }
+  const uint64_t nt = (n + kTile - 1) / kTile;
+  hipLaunchKernelGGL(k_tile_sums, dim3(nt), dim3(kT), 0, s, lengths, n, tiles);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kT), 0, s, tiles, nt, start);
+  hipLaunchKernelGGL(k_apply, dim3(nt), dim3(kT), 0, s, lengths, n, tiles, offsets);
+  return hipGetLastError();
+}
+
+hipError_t zx_launch_fill(uint64_t *words, uint64_t n, uint64_t first, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, words, n, first);
+  return hipGetLastError();
+}

@@ -1,0 +1,106 @@
+"""Seeded synthetic fields for benchmarks and tests.
+
+grf(shape, seed): a Gaussian random field with power spectrum ~ k^-3
+(spectral synthesis: white noise filtered by k^-3/2 in Fourier space), scaled
+to unit variance, plus 1e-3 white noise.  Optionally 5 % of the 4^d blocks set
+to zero and 1 % of values replaced by subnormals, as the large fixed-rate
+workload specifies.  Generated in float64 (torch on the GPU when one is given,
+numpy otherwise) and cast once to the requested dtype.
+"""
+import numpy as np
+
+
+def _grf_numpy(shape, seed):
+    rng = np.random.default_rng(seed)
+    noise = rng.standard_normal(shape)
+    spec = np.fft.rfftn(noise)
+    k2 = np.zeros(spec.shape)
+    for ax, n in enumerate(shape):
+        if ax == len(shape) - 1:
+            k = np.fft.rfftfreq(n) * n
+        else:
+            k = np.fft.fftfreq(n) * n
+        sh = [1] * len(shape)
+        sh[ax] = k.size
+        k2 = k2 + (k.reshape(sh)) ** 2
+    k2[(0,) * len(shape)] = 1.0
+    spec *= k2 ** (-3.0 / 4.0)  # amplitude ~ k^-3/2 -> power ~ k^-3
+    spec[(0,) * len(shape)] = 0.0
+    f = np.fft.irfftn(spec, s=shape)
+    f /= f.std()
+    f += 1e-3 * rng.standard_normal(shape)
+    return f
+
+
+def _grf_torch(shape, seed, device):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    noise = torch.randn(shape, generator=g, device=device, dtype=torch.float64)
+    spec = torch.fft.rfftn(noise)
+    k2 = torch.zeros(spec.shape, device=device, dtype=torch.float64)
+    for ax, n in enumerate(shape):
+        if ax == len(shape) - 1:
+            k = torch.fft.rfftfreq(n, device=device, dtype=torch.float64) * n
+        else:
+            k = torch.fft.fftfreq(n, device=device, dtype=torch.float64) * n
+        sh = [1] * len(shape)
+        sh[ax] = k.numel()
+        k2 = k2 + k.reshape(sh) ** 2
+    del noise
+    k2[(0,) * len(shape)] = 1.0
+    spec *= k2.pow(-0.75)
+    del k2
+    spec[(0,) * len(shape)] = 0.0
+    f = torch.fft.irfftn(spec, s=shape)
+    del spec
+    f /= f.std()
+    f += 1e-3 * torch.randn(shape, generator=g, device=device, dtype=torch.float64)
+    return f
+
+
+def grf(shape, seed, dtype=np.float32, zero_blocks=0.0, subnormals=0.0, scale=1.0, device=None):
+    """Return a numpy array (device=None) or a torch tensor on `device`."""
+    shape = tuple(int(n) for n in shape)
+    if device is None:
+        f = _grf_numpy(shape, seed) * scale
+        rng = np.random.default_rng(seed + 1)
+        out = f.astype(dtype)
+        if zero_blocks > 0:
+            nb = [(n + 3) // 4 for n in shape]
+            total = int(np.prod(nb))
+            pick = rng.choice(total, size=int(total * zero_blocks), replace=False)
+            coords = np.unravel_index(pick, nb)
+            for c in zip(*coords):
+                out[tuple(slice(4 * i, 4 * i + 4) for i in c)] = 0
+        if subnormals > 0 and np.dtype(dtype).kind == "f":
+            tiny = np.finfo(dtype).tiny
+            idx = rng.choice(out.size, size=int(out.size * subnormals), replace=False)
+            flat = out.reshape(-1)
+            flat[idx] = (rng.uniform(-1, 1, size=idx.size) * tiny).astype(dtype)
+        return out
+    import torch
+    f = _grf_torch(shape, seed, device)
+    if scale != 1.0:
+        f *= scale
+    tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32,
+           np.int64: torch.int64}[np.dtype(dtype).type]
+    out = f.to(tdt)
+    del f
+    g = torch.Generator(device=device)
+    g.manual_seed(seed + 1)
+    if zero_blocks > 0:
+        nb = [(n + 3) // 4 for n in shape]
+        mask = torch.rand(nb, generator=g, device=device) < zero_blocks
+        for ax in range(len(shape)):
+            mask = mask.repeat_interleave(4, dim=ax)
+        mask = mask[tuple(slice(0, n) for n in shape)]
+        out[mask] = 0
+        del mask
+    if subnormals > 0 and out.is_floating_point():
+        tiny = torch.finfo(out.dtype).tiny
+        sel = torch.rand(shape, generator=g, device=device) < subnormals
+        vals = (torch.rand(shape, generator=g, device=device, dtype=torch.float64) * 2 - 1) * tiny
+        out[sel] = vals[sel].to(out.dtype)
+        del sel, vals
+    return out
