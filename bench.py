@@ -1,0 +1,187 @@
+"""Benchmark: fixed-rate 512^3 f32 ZFP compress + decompress on one MI355X.
+
+A step is one compress and one decompress of the whole field through the C
+ABI (zfp_compress / zfp_decompress), field and stream resident in HBM.  value
+is uncompressed GB/s over the step: 2 x 512 MiB of field data handled per
+step (compress reads it, decompress writes it) / wall time per step.
+
+Also reported:
+  roofline      the dominant kernel (the slower of encode and decode):
+                algorithmic bytes per launch (field bytes + stream bytes) /
+                its average duration from HIP events recorded on the stream
+                the kernel runs on, against MI355X's 8 TB/s HBM peak;
+  cpu_baseline  the oracle's threaded CPU codec (C-OMP chunked compress,
+                per-chunk-seek fixed-rate decompress) on a 512x512x64 slab of
+                the same field, timed on this host, and its stream checked
+                byte for byte against the GPU's.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rate 8] [--n 512]
+"""
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+for p in (ROOT / "zfp-rs_amd", ROOT / "oracle"):
+    sys.path.insert(0, str(p))
+
+METRIC = "compress+decompress GB/s (uncompressed), 512^3 f32 rate 8; batched p99 ms"
+HBM_PEAK_GBPS = 8000.0
+
+
+def cpu_baseline(field_gpu, rate, gpu_words_fn, budget_s=12.0):
+    """Time the oracle (threaded) on a slab; returns the cpu_baseline object."""
+    import numpy as np
+
+    import oracle_py as O
+
+    slab = field_gpu[:64].contiguous().cpu().numpy()
+    cfg = O.fixed_rate(rate, O.FLOAT, 3)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    # the GPU stream of the same slab must be byte-identical
+    on, oerr, ow = O.compress(slab, cfg, threads=threads)
+    gw = gpu_words_fn(slab)
+    same = oerr == 0 and np.array_equal(gw[: on // 8], ow[: on // 8])
+    best = None
+    t_end = time.perf_counter() + budget_s
+    reps = 0
+    while reps < 2 or (time.perf_counter() < t_end and reps < 20):
+        t0 = time.perf_counter()
+        n, err, words = O.compress(slab, cfg, threads=threads)
+        t1 = time.perf_counter()
+        out, m, derr = O.decompress(words, slab.shape, slab.dtype, cfg, threads=threads)
+        t2 = time.perf_counter()
+        assert err == 0 and derr == 0
+        step = t2 - t0
+        best = step if best is None or step < best else best
+        reps += 1
+    gbps = 2 * slab.nbytes / best / 1e9
+    import platform
+    cpu_model = platform.processor() or "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": round(gbps, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"512x512x64 f32 slab of the same field, rate {rate}, compress+decompress, "
+                      f"best of {reps}, oracle threaded C code, {threads} threads on {cpu_model} "
+                      f"(nproc {os.cpu_count()})",
+            "stream_matches_gpu": bool(same)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--rate", type=float, default=8.0)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X "
+              "(see DESIGN.md for how the codec would shard); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import zfp_hip as Z
+    from synth import grf
+
+    torch.cuda.set_device(0)
+    n = args.n
+    shape = (n, n, n)
+    x = grf(shape, seed=2024, zero_blocks=0.05, subnormals=0.01, device="cuda")
+    assert x.dtype == torch.float32 and x.is_contiguous()
+    out = torch.empty_like(x)
+    cfg = Z.ZfpConfig.fixed_rate(args.rate, Z.TYPE_FLOAT, 3)
+    nbytes_max = Z.maximum_size(cfg, x)
+    buf = torch.zeros((nbytes_max + 7) // 8, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+    s = Z.Stream(buf, cfg)
+    s.set_hip_stream(stream.cuda_stream)
+    s.set_timing(True)
+
+    enc_ms, dec_ms = [], []
+
+    def step(record):
+        s.rewind()
+        c = s.compress(x)
+        if record:
+            enc_ms.append(s.kernel_ms()[0])
+        s.rewind()
+        d = s.decompress(out)
+        if record:
+            dec_ms.append(s.kernel_ms()[0])
+        return c, d
+
+    for _ in range(args.warmup):
+        c_bytes, d_bytes = step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        c_bytes, d_bytes = step(True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+
+    field_bytes = x.numel() * 4
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = 2 * field_bytes / (ms_per_step * 1e-3) / 1e9
+    stream_bytes = c_bytes
+    enc = float(np.mean(enc_ms))
+    dec = float(np.mean(dec_ms))
+    alg = field_bytes + stream_bytes  # per launch, either direction
+    dominant = "encode" if enc >= dec else "decode"
+    dom_ms = max(enc, dec)
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    # sanity: a decode of the compressed stream round-trips the block count
+    blocks = ((n + 3) // 4) ** 3
+    assert stream_bytes == (blocks * cfg.maxbits + 63) // 64 * 8
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded k^-3 Gaussian random field, unit variance + 1e-3 noise, "
+                "5% zero blocks, 1% subnormals; field and stream resident in HBM",
+        "config": {"workload": f"{n}^3 f32 fixed-rate {args.rate:g} compress+decompress "
+                               "(zfp_compress/zfp_decompress, device-resident)",
+                   "rate": args.rate, "shape": list(shape), "rounding": "Never",
+                   "stream_bytes": stream_bytes},
+        "compress_GBps": round(field_bytes / (enc * 1e-3) / 1e9, 2),
+        "decompress_GBps": round(field_bytes / (dec * 1e-3) / 1e9, 2),
+        "encode_kernel_ms": round(enc, 4),
+        "decode_kernel_ms": round(dec, 4),
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": alg},
+        "cpu_baseline": None,
+    }
+
+    if not args.no_cpu_baseline:
+        def gpu_words(slab):
+            b, nb = Z.compress(slab, cfg)
+            return b
+        result["cpu_baseline"] = cpu_baseline(x, args.rate, gpu_words)
+    s.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -32,6 +32,13 @@ int zfp_stream_rounding(const zfp_stream *stream, zfp_bool *tight_error);
  * stream.  zfp_compress/zfp_decompress still return only when done. */
 zfp_bool zfp_stream_set_hip_stream(zfp_stream *stream, void *hip_stream);
 
+/* Kernel timing: when on, the codec records HIP events on its stream around
+ * its kernels; zfp_stream_kernel_ms returns the block-coding kernel's time of
+ * the last compress/decompress in milliseconds (and, through call_ms, the
+ * time from its first to its last kernel), or -1 when timing is off. */
+zfp_bool zfp_stream_set_timing(zfp_stream *stream, zfp_bool on);
+double zfp_stream_kernel_ms(const zfp_stream *stream, double *call_ms);
+
 /* Offset index of the last variable-rate compression through a bit stream:
  * one uint32 bit length per block, in block order, for the blocks written
  * from bit `start_bit`.  zfp_decompress uses a matching index (same start,

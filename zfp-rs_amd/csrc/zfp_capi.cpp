@@ -546,6 +546,39 @@ struct ExecParams {
   int32_t rounding;  // zx::ROUND_*
   int32_t tight;
   hipStream_t hip;
+  // kernel timing (zfp_stream_set_timing): events around the main kernel
+  int32_t timing;
+  hipEvent_t ev[4];
+  float kernel_ms;   // the block-coding kernel of the last call
+  float call_ms;     // every kernel of the last call
+};
+
+// Records timing events on `st` when the stream asked for timing.
+struct Timer {
+  ExecParams *p;
+  hipStream_t st;
+  bool on;
+  Timer(ExecParams *pp, hipStream_t s) : p(pp), st(s), on(pp && pp->timing)
+  {
+    if (on && !p->ev[0]) {
+      for (int i = 0; i < 4; i++)
+        if (hipEventCreate(&p->ev[i]) != hipSuccess) { (void)hipGetLastError(); on = false; }
+    }
+  }
+  void mark(int i)
+  {
+    if (on) (void)hipEventRecord(p->ev[i], st);
+  }
+  // after the stream was synchronized
+  void collect()
+  {
+    if (!on) return;
+    float a = 0, b = 0;
+    if (hipEventElapsedTime(&a, p->ev[1], p->ev[2]) != hipSuccess) { (void)hipGetLastError(); a = -1; }
+    if (hipEventElapsedTime(&b, p->ev[0], p->ev[3]) != hipSuccess) { (void)hipGetLastError(); b = -1; }
+    p->kernel_ms = a;
+    p->call_ms = b;
+  }
 };
 
 ExecParams *params(zfp_stream *z, bool create)
@@ -734,7 +767,8 @@ bool stage_field(Workspace &W, const zfp_field *f, const Plan &p, bool copy_in, 
 // Encode every block of the field at the cursor of `s`; leaves the cursor
 // where compress + finish would (compress.rs:22-53).  Returns false on a
 // device failure (nothing consistent was written).
-bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st)
+bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st,
+                  ExecParams *xp)
 {
   Workspace &W = ws();
   std::lock_guard<std::mutex> lock(W.mu);
@@ -744,7 +778,9 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   const bool rev = c.minexp < kMinExp;
   void *dorigin = nullptr;
   bool staged = false;
+  Timer tm(xp, st);
   if (!stage_field(W, f, p, true, st, dorigin, staged)) return false;
+  tm.mark(0);
   zx::EncodeArgs a;
   a.field = field_desc(p, dorigin);
   a.cfg = c;
@@ -790,7 +826,9 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   a.nwords = std::min(cap_words, live);
   const bool aligned = a.start == 0 && fixed && (c.maxbits % 64) == 0;
   if (!aligned && live) ZCHECK(zx_launch_fill(dwords, live, prefix, st));
+  tm.mark(1);
   if (live) ZCHECK(launch_encode(p.type, p.dims, rev, a, st));
+  tm.mark(2);
   if (!s->device && live) {
     ZCHECK(hipMemcpyAsync(s->words + w0, dwords, live * 8, hipMemcpyDeviceToHost, st));
   }
@@ -812,7 +850,9 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
       s->index_key = cfg_key(c, p);
     }
   }
+  tm.mark(3);
   ZCHECK(hipStreamSynchronize(st));
+  tm.collect();
   // the cursor after the last block, then flush (core.rs:536-542)
   if (w1 > s->nwords) s->overflowed = true;
   s->word_pos = (size_t)w1;
@@ -824,7 +864,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
 // Decode every block of the field from the cursor of `s`; leaves the cursor
 // and overread state where decompress + finish would (decompress.rs:27-74).
 bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st,
-                    uint64_t &end_out)
+                    ExecParams *xp, uint64_t &end_out)
 {
   Workspace &W = ws();
   std::lock_guard<std::mutex> lock(W.mu);
@@ -835,7 +875,9 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   const size_t seek_word0 = s->seek_word;
   void *dorigin = nullptr;
   bool staged = false;
+  Timer tm(xp, st);
   if (!stage_field(W, f, p, !contiguous(p), st, dorigin, staged)) return false;
+  tm.mark(0);
   zx::DecodeArgs a;
   a.field = field_desc(p, dorigin);
   a.cfg = c;
@@ -889,14 +931,18 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   }
   if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
   ZCHECK(zx_launch_fill(maxword, 1, 0, st));
+  tm.mark(1);
   ZCHECK(launch_decode(p.type, p.dims, rev, a, nullptr, st));
+  tm.mark(2);
   ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 1, maxword, 8, hipMemcpyDeviceToHost, st));
   if (staged) {
     const size_t span = (size_t)(p.imax - p.imin + 1) * p.esz;
     char *lo = (char *)f->data + p.imin * (int64_t)p.esz;
     ZCHECK(hipMemcpyAsync(lo, W.field, span, hipMemcpyDeviceToHost, st));
   }
+  tm.mark(3);
   ZCHECK(hipStreamSynchronize(st));
+  tm.collect();
   if (!fixed) end_rel = *(uint64_t *)W.one;
   const uint64_t end = 64 * w0 + end_rel;
   end_out = end;
@@ -980,6 +1026,9 @@ zfp_stream *zfp_stream_open(bitstream *bs)
 void zfp_stream_close(zfp_stream *z)
 {
   if (!z) return;
+  if (ExecParams *p = params(z, false))
+    for (int i = 0; i < 4; i++)
+      if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
   free(z->exec.params);
   free(z);
 }
@@ -1177,7 +1226,7 @@ size_t zfp_compress(zfp_stream *z, const zfp_field *f)
   bitstream *s = z->stream;
   const zx::Cfg c = cfg_of(z);
   if (p.nblocks > 0) {
-    if (!gpu_compress(s, f, p, c, hip_stream_of(z))) return 0;
+    if (!gpu_compress(s, f, p, c, hip_stream_of(z), params(z, false))) return 0;
   } else {
     s->flush();
   }
@@ -1198,7 +1247,7 @@ size_t zfp_decompress(zfp_stream *z, zfp_field *f)
   const zx::Cfg c = cfg_of(z);
   if (p.nblocks > 0) {
     uint64_t end;
-    if (!gpu_decompress(s, f, p, c, hip_stream_of(z), end)) return 0;
+    if (!gpu_decompress(s, f, p, c, hip_stream_of(z), params(z, false), end)) return 0;
   } else {
     s->reset_overread();
     s->align();
@@ -1516,6 +1565,22 @@ zfp_bool zfp_stream_index_import(bitstream *s, const uint32_t *lengths, size_t c
   s->index_start = start;
   s->index_key = 0;  // an imported index matches on start and block count
   return zfp_true;
+}
+
+zfp_bool zfp_stream_set_timing(zfp_stream *z, zfp_bool on)
+{
+  if (!z) return zfp_false;
+  ExecParams *p = params(z, true);
+  if (!p) return zfp_false;
+  p->timing = on ? 1 : 0;
+  return zfp_true;
+}
+
+double zfp_stream_kernel_ms(const zfp_stream *z, double *call_ms)
+{
+  const ExecParams *p = z ? params(z) : nullptr;
+  if (call_ms) *call_ms = p && p->timing ? p->call_ms : -1.0;
+  return p && p->timing ? p->kernel_ms : -1.0;
 }
 
 int zfp_hip_device(void)

@@ -122,6 +122,8 @@ SIGNATURES = {
     "zfp_stream_set_hip_stream": (_B, [_SP, _VP]),
     "zfp_stream_index_export": (_SZ, [_VP, P(ctypes.c_uint32), _SZ, P(_U64)]),
     "zfp_stream_index_import": (_B, [_VP, P(ctypes.c_uint32), _SZ, _U64]),
+    "zfp_stream_set_timing": (_B, [_SP, _B]),
+    "zfp_stream_kernel_ms": (_D, [_SP, P(_D)]),
     "zfp_hip_device": (ctypes.c_int, []),
 }
 for _name in ("zfp_config_none", "zfp_config_rate", "zfp_config_precision", "zfp_config_accuracy",
@@ -282,6 +284,15 @@ class Stream:
 
     def set_hip_stream(self, handle):
         lib().zfp_stream_set_hip_stream(self.zs, handle)
+
+    def set_timing(self, on=True):
+        lib().zfp_stream_set_timing(self.zs, int(on))
+
+    def kernel_ms(self):
+        """(block-coding kernel ms, all-kernels ms) of the last call, HIP-event timed."""
+        call = ctypes.c_double(0)
+        k = lib().zfp_stream_kernel_ms(self.zs, ctypes.byref(call))
+        return k, call.value
 
     def compress(self, arr):
         """Bytes of stream after the flush (ops.rs:218-224), or BufferTooSmall."""
