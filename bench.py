@@ -83,6 +83,8 @@ def main():
     ap.add_argument("--rate", type=float, default=8.0)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one device-to-device copy of the field (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X "
@@ -130,6 +132,10 @@ def main():
         c_bytes, d_bytes = step(True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+
+    if args.calib_copy:
+        out.copy_(x)
+        torch.cuda.synchronize()
 
     field_bytes = x.numel() * 4
     ms_per_step = (t1 - t0) * 1e3 / args.steps

@@ -24,12 +24,13 @@ def lib():
         L.hc_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                   P(ctypes.c_int64), P(ctypes.c_int64), P(HcCfg),
                                   P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
-                                  ctypes.c_uint32, P(ctypes.c_uint32)]
+                                  ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32]
         L.hc_compress.restype = ctypes.c_uint64
         L.hc_decompress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                     P(ctypes.c_int64), P(ctypes.c_int64), P(HcCfg),
                                     P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
-                                    ctypes.c_uint32, P(ctypes.c_uint32)]
+                                    ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32]
+        L.hc_decompress.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -44,7 +45,8 @@ def cfg_of(oc):
     return HcCfg(oc.minbits, oc.maxbits, oc.maxprec, oc.minexp, oc.rounding, oc.tight)
 
 
-def compress(arr, ztype, oc, words, start_bit, fixed_bits, rev):
+def compress(arr, ztype, oc, words, start_bit, fixed_bits, rev, lds_T=0):
+    """lds_T > 0 runs the fixed-rate LDS path with workgroups of lds_T blocks."""
     n, s = _dims(arr)
     nb = 1
     for v in list(reversed(arr.shape)):
@@ -54,16 +56,18 @@ def compress(arr, ztype, oc, words, start_bit, fixed_bits, rev):
     end = lib().hc_compress(ztype, arr.ndim, int(rev), arr.ctypes.data, n, s, ctypes.byref(c),
                             words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
                             start_bit, fixed_bits,
-                            lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+                            lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), lds_T)
     return end, lens
 
 
-def decompress(shape, dtype, ztype, oc, words, start_bit, fixed_bits, lens, rev):
+def decompress(shape, dtype, ztype, oc, words, start_bit, fixed_bits, lens, rev, lds_T=0):
+    """Returns (values, loaded) where loaded is one past the highest word read
+    (LDS path only; 0 otherwise)."""
     out = np.zeros(shape, dtype=dtype)
     n, s = _dims(out)
     c = cfg_of(oc)
     lp = lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if lens is not None else None
-    lib().hc_decompress(ztype, out.ndim, int(rev), out.ctypes.data, n, s, ctypes.byref(c),
-                        words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
-                        start_bit, fixed_bits, lp)
-    return out
+    loaded = lib().hc_decompress(ztype, out.ndim, int(rev), out.ctypes.data, n, s, ctypes.byref(c),
+                                 words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
+                                 start_bit, fixed_bits, lp, lds_T)
+    return out, loaded

@@ -5,7 +5,7 @@
 #include <stdlib.h>
 #include <vector>
 
-#include "zfpx_field.h"
+#include "zfpx_fx.h"
 
 using namespace zx;
 
@@ -66,16 +66,74 @@ static void decompress_t(const FieldDesc &f, const Cfg &c, const uint64_t *words
   uint64_t off = start;
   for (uint64_t b = 0; b < f.nblocks; b++) {
     WordSource r;
-    r.init(words, nwords, off);
+    r.init(GlobalWords{words, nwords}, off);
     decode_field_block<S, D, REV>(f, b, r, c);
     off += fixed ? fixed : lens[b];
   }
 }
 
+// The fixed-rate LDS path of the kernels (zfpx_fx.h), one workgroup of T
+// threads at a time, each phase run for every thread before the next.
+template <class S, int D, bool REV>
+static uint64_t compress_fx(const FieldDesc &f, const Cfg &c, uint64_t *words, uint64_t nwords,
+                            uint64_t start, uint32_t fixed, const FxPlan &p)
+{
+  typedef typename Ints<ST<S>::P>::U U;
+  std::vector<uint64_t> lds(p.lds_words);
+  const uint64_t groups = (f.nblocks + p.T - 1) / p.T;
+  for (uint64_t g = 0; g < groups; g++) {
+    const FxGroup q = fx_group(g, p.T, f.nblocks, start, fixed);
+    for (auto &x : lds) x = 0x5555aaaa5555aaaaull;  // LDS starts undefined
+    if (!p.own)
+      for (uint32_t i = 0; i < q.nw; i++) fx_enc_clear(lds.data(), p, i);
+    for (uint32_t t = 0; t < q.nb; t++) {
+      U v[Blk<D>::SIZE];
+      gather<S, D>(f, block_pos<D>(f, q.b0 + t), v);
+      if (p.own) fx_enc_block<S, D, REV, true>(lds.data(), p, q, t, fixed, v, c);
+      else fx_enc_block<S, D, REV, false>(lds.data(), p, q, t, fixed, v, c);
+    }
+    for (uint32_t i = 0; i < q.nw; i++) {
+      const bool in = p.own ? fx_enc_store<true>(words, nwords, lds.data(), p, q, i, fixed)
+                            : fx_enc_store<false>(words, nwords, lds.data(), p, q, i, fixed);
+      if (!in) break;
+    }
+  }
+  return start + f.nblocks * fixed;
+}
+
+template <class S, int D, bool REV>
+static uint64_t decompress_fx(const FieldDesc &f, const Cfg &c, const uint64_t *words,
+                              uint64_t nwords, uint64_t start, uint32_t fixed, const FxPlan &p)
+{
+  typedef typename Ints<ST<S>::P>::U U;
+  std::vector<uint64_t> lds(p.lds_words, 0x5555aaaa5555aaaaull);
+  const uint64_t groups = (f.nblocks + p.T - 1) / p.T;
+  uint64_t maxword = 0;
+  for (uint64_t g = 0; g < groups; g++) {
+    const FxGroup q = fx_group(g, p.T, f.nblocks, start, fixed);
+    for (uint32_t i = 0; i < q.nw + 1; i++) fx_dec_stage(lds.data(), words, nwords, p, q, i);
+    for (uint32_t t = 0; t < q.nb; t++) {
+      U v[Blk<D>::SIZE];
+      const uint64_t loaded = fx_dec_block<S, D, REV>(lds.data(), p, q, t, fixed, v, c);
+      if (q.b0 + t == f.nblocks - 1) maxword = loaded;
+      scatter<S, D>(f, block_pos<D>(f, q.b0 + t), v);
+    }
+  }
+  return maxword;
+}
+
 template <class S, bool REV>
 static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, uint64_t nw,
-                       uint64_t st, uint32_t fx, uint32_t *l)
+                       uint64_t st, uint32_t fx, uint32_t *l, uint32_t T)
 {
+  FxPlan p;
+  if (T && fx_plan(fx, st, false, T, p)) {
+    switch (dims) {
+    case 1: return compress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
+    case 2: return compress_fx<S, 2, REV>(f, c, w, nw, st, fx, p);
+    default: return compress_fx<S, 3, REV>(f, c, w, nw, st, fx, p);
+    }
+  }
   switch (dims) {
   case 1: return compress_t<S, 1, REV>(f, c, w, nw, st, fx, l);
   case 2: return compress_t<S, 2, REV>(f, c, w, nw, st, fx, l);
@@ -84,51 +142,60 @@ static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, 
 }
 
 template <class S, bool REV>
-static void decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint64_t *w, uint64_t nw,
-                     uint64_t st, uint32_t fx, const uint32_t *l)
+static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint64_t *w, uint64_t nw,
+                         uint64_t st, uint32_t fx, const uint32_t *l, uint32_t T)
 {
+  FxPlan p;
+  if (T && fx_plan(fx, st, true, T, p)) {
+    switch (dims) {
+    case 1: return decompress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
+    case 2: return decompress_fx<S, 2, REV>(f, c, w, nw, st, fx, p);
+    default: return decompress_fx<S, 3, REV>(f, c, w, nw, st, fx, p);
+    }
+  }
   switch (dims) {
   case 1: decompress_t<S, 1, REV>(f, c, w, nw, st, fx, l); break;
   case 2: decompress_t<S, 2, REV>(f, c, w, nw, st, fx, l); break;
   default: decompress_t<S, 3, REV>(f, c, w, nw, st, fx, l); break;
   }
+  return 0;
 }
 
 extern "C" {
 
 uint64_t hc_compress(int type, int dims, int rev, void *data, const int64_t *n, const int64_t *s,
                      const HcCfg *hc, uint64_t *words, uint64_t nwords, uint64_t start_bit,
-                     uint32_t fixed_bits, uint32_t *lengths)
+                     uint32_t fixed_bits, uint32_t *lengths, uint32_t lds_T)
 {
   FieldDesc f = make_field(dims, data, n, s);
   Cfg c{hc->minbits, hc->maxbits, hc->maxprec, hc->minexp, hc->rounding, hc->tight};
   switch (type * 2 + (rev ? 1 : 0)) {
-  case 2: return comp_d<int32_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
-  case 3: return comp_d<int32_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
-  case 4: return comp_d<int64_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
-  case 5: return comp_d<int64_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
-  case 6: return comp_d<float, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
-  case 7: return comp_d<float, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
-  case 8: return comp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
-  default: return comp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths);
+  case 2: return comp_d<int32_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 3: return comp_d<int32_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 4: return comp_d<int64_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 5: return comp_d<int64_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 6: return comp_d<float, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 7: return comp_d<float, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 8: return comp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  default: return comp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
   }
 }
 
-void hc_decompress(int type, int dims, int rev, void *data, const int64_t *n, const int64_t *s,
+uint64_t hc_decompress(int type, int dims, int rev, void *data, const int64_t *n, const int64_t *s,
                    const HcCfg *hc, const uint64_t *words, uint64_t nwords, uint64_t start_bit,
-                   uint32_t fixed_bits, const uint32_t *lengths)
+                   uint32_t fixed_bits, const uint32_t *lengths, uint32_t lds_T)
 {
   FieldDesc f = make_field(dims, data, n, s);
   Cfg c{hc->minbits, hc->maxbits, hc->maxprec, hc->minexp, hc->rounding, hc->tight};
   switch (type * 2 + (rev ? 1 : 0)) {
-  case 2: decomp_d<int32_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
-  case 3: decomp_d<int32_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
-  case 4: decomp_d<int64_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
-  case 5: decomp_d<int64_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
-  case 6: decomp_d<float, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
-  case 7: decomp_d<float, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
-  case 8: decomp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
-  default: decomp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths); break;
+  case 2: return decomp_d<int32_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 3: return decomp_d<int32_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 4: return decomp_d<int64_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 5: return decomp_d<int64_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 6: return decomp_d<float, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 7: return decomp_d<float, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 8: return decomp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  default: return decomp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
   }
 }
 

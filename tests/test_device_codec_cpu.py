@@ -58,23 +58,53 @@ def run_case(arr, ztype, cfg, start_bits):
     n, err, ow = O.compress(arr, cfg, start_bits=start_bits)
     assert err == 0
     nwords = n // 8
-    words = np.zeros(nwords + 4, dtype=np.uint64)
-    words[: start_bits // 64] = PREFIX_WORD
-    if start_bits % 64:
-        words[start_bits // 64] = PREFIX_PART & ((1 << (start_bits % 64)) - 1)
     fixed = cfg.maxbits if is_fixed(cfg, ztype, dims) else 0
     rev = cfg.minexp < -1074
-    end, lens = H.compress(arr, ztype, cfg, words, start_bits, fixed, rev)
-    assert (end + 63) // 64 * 8 == n, "stream length"
-    assert np.array_equal(words[:nwords], ow[:nwords]), "stream words"
-    assert not words[nwords:].any()
     ref, rn, rerr = O.decompress(ow, arr.shape, arr.dtype, cfg, start_bits=start_bits)
     assert rerr == 0
-    got = H.decompress(arr.shape, arr.dtype, ztype, cfg, words, start_bits, fixed,
-                       None if fixed else lens, rev)
     isz = arr.dtype.itemsize
     u = np.uint32 if isz == 4 else np.uint64
-    assert np.array_equal(got.view(u), ref.view(u)), "decoded values"
+    # the global-memory path, and for fixed-rate streams the LDS path with
+    # workgroups of 3 blocks (group boundaries inside the field) and of 64
+    for lds_T in ((0, 3, 64) if fixed and fixed <= 4096 else (0,)):
+        words = np.zeros(nwords + 4, dtype=np.uint64)
+        words[: start_bits // 64] = PREFIX_WORD
+        if start_bits % 64:
+            words[start_bits // 64] = PREFIX_PART & ((1 << (start_bits % 64)) - 1)
+        end, lens = H.compress(arr, ztype, cfg, words, start_bits, fixed, rev, lds_T)
+        assert (end + 63) // 64 * 8 == n, f"stream length (lds_T={lds_T})"
+        assert np.array_equal(words[:nwords], ow[:nwords]), f"stream words (lds_T={lds_T})"
+        assert not words[nwords:].any()
+        got, loaded = H.decompress(arr.shape, arr.dtype, ztype, cfg, words, start_bits, fixed,
+                                   None if fixed else lens, rev, lds_T)
+        assert np.array_equal(got.view(u), ref.view(u)), f"decoded values (lds_T={lds_T})"
+        if lds_T:
+            # a whole stream is read without loading past its end
+            assert 0 < loaded <= nwords, f"loaded words (lds_T={lds_T})"
+
+
+def test_lds_path_capacity_and_overread():
+    """Fixed-rate LDS path against a stream cut short: writes past the
+    capacity are dropped, reads past it return zero bits (bitstream/core.rs
+    write and read at the end of the buffer)."""
+    arr = smooth((8, 8, 12), np.float32, 5, scale=10.0)
+    for rate, start in ((8, 0), (5, 0), (5, 13)):
+        cfg = O.fixed_rate(rate, O.FLOAT, 3)
+        n, err, ow = O.compress(arr, cfg, start_bits=start)
+        cut = n // 8 - 3
+        ref_words = ow[:cut].copy()
+        on, oerr, ow2 = O.compress(arr, cfg, capacity=cut * 8, start_bits=start)
+        words = np.zeros(cut, dtype=np.uint64)
+        words[: start // 64] = PREFIX_WORD
+        if start % 64:
+            words[start // 64] = PREFIX_PART & ((1 << (start % 64)) - 1)
+        H.compress(arr, O.FLOAT, cfg, words, start, cfg.maxbits, False, 4)
+        assert np.array_equal(words, ow2[:cut])
+        ref, _, _ = O.decompress(ref_words, arr.shape, arr.dtype, cfg, start_bits=start)
+        got, loaded = H.decompress(arr.shape, arr.dtype, O.FLOAT, cfg, ref_words, start,
+                                   cfg.maxbits, None, False, 4)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+        assert loaded > cut
 
 
 @pytest.mark.parametrize("dims", [1, 2, 3])

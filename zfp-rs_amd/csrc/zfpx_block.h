@@ -241,109 +241,129 @@ template <int D, class I, class L> ZX_HD void inv_xform(I (&p)[Blk<D>::SIZE], L 
   lift_axis<Blk<D>::SIZE, 1>(p, lift);
 }
 
+// ---------------------------------------------------------------- wave votes
+
+// True if the predicate holds on any lane of the wave (on the host: this lane).
+ZX_HD bool wave_any(bool p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ballot(p) != 0;
+#else
+  return p;
+#endif
+}
+
 // ---------------------------------------------------------------- bit planes
 
-// 32x32 bit-matrix transpose: row r bit c <-> row c bit r (quadrant swaps).
-ZX_HD void tr32(uint32_t (&a)[32])
+// Stages of the 32x32 bit-matrix transpose (row r bit c <-> row c bit r).
+// The stage of width j swaps index bit log2(j) of rows and columns; the five
+// stages commute, so any subset of rows can be finished independently.
+template <int J, int N>
+ZX_HD void tr_stage(uint32_t (&a)[N])
+{
+  constexpr uint32_t M = J == 8 ? 0x00ff00ffu : J == 4 ? 0x0f0f0f0fu : J == 2 ? 0x33333333u : 0x55555555u;
+  ZX_UNROLL
+  for (int k = 0; k < N; k += 2 * J) {
+    ZX_UNROLL
+    for (int i = k; i < k + J; i++) {
+      uint32_t lo = a[i], hi = a[i + J];
+      a[i] = (lo & M) | ((hi << J) & ~M);
+      a[i + J] = ((lo >> J) & M) | (hi & ~M);
+    }
+  }
+}
+
+// rows 16..31 (HI) or 0..15 (!HI) of the transpose of a[32]
+template <bool HI>
+ZX_HD void tr32_half(const uint32_t (&a)[32], uint32_t (&o)[16])
 {
   ZX_UNROLL
-  for (int i = 0; i < 16; i++) {
-    uint32_t lo = a[i], hi = a[i + 16];
-    a[i] = (lo & 0x0000ffffu) | (hi << 16);
-    a[i + 16] = (lo >> 16) | (hi & 0xffff0000u);
-  }
-  ZX_UNROLL
-  for (int k = 0; k < 32; k += 16) {
-    ZX_UNROLL
-    for (int i = k; i < k + 8; i++) {
-      uint32_t lo = a[i], hi = a[i + 8];
-      a[i] = (lo & 0x00ff00ffu) | ((hi << 8) & 0xff00ff00u);
-      a[i + 8] = ((lo >> 8) & 0x00ff00ffu) | (hi & 0xff00ff00u);
-    }
-  }
-  ZX_UNROLL
-  for (int k = 0; k < 32; k += 8) {
-    ZX_UNROLL
-    for (int i = k; i < k + 4; i++) {
-      uint32_t lo = a[i], hi = a[i + 4];
-      a[i] = (lo & 0x0f0f0f0fu) | ((hi << 4) & 0xf0f0f0f0u);
-      a[i + 4] = ((lo >> 4) & 0x0f0f0f0fu) | (hi & 0xf0f0f0f0u);
-    }
-  }
-  ZX_UNROLL
-  for (int k = 0; k < 32; k += 4) {
-    ZX_UNROLL
-    for (int i = k; i < k + 2; i++) {
-      uint32_t lo = a[i], hi = a[i + 2];
-      a[i] = (lo & 0x33333333u) | ((hi << 2) & 0xccccccccu);
-      a[i + 2] = ((lo >> 2) & 0x33333333u) | (hi & 0xccccccccu);
-    }
-  }
-  ZX_UNROLL
-  for (int i = 0; i < 32; i += 2) {
-    uint32_t lo = a[i], hi = a[i + 1];
-    a[i] = (lo & 0x55555555u) | ((hi << 1) & 0xaaaaaaaau);
-    a[i + 1] = ((lo >> 1) & 0x55555555u) | (hi & 0xaaaaaaaau);
-  }
+  for (int i = 0; i < 16; i++)
+    o[i] = HI ? (a[i] >> 16) | (a[i + 16] & 0xffff0000u) : (a[i] & 0x0000ffffu) | (a[i + 16] << 16);
+  tr_stage<8>(o);
+  tr_stage<4>(o);
+  tr_stage<2>(o);
+  tr_stage<1>(o);
 }
 
 // Plane word: one bit per coefficient.
 template <int SIZE> struct PWord { typedef uint32_t T; };
 template <> struct PWord<64> { typedef uint64_t T; };
 
-// planes[k] bit i = bit k of coefficient i
+// Planes 16c..16c+15 of a block: out[i] bit j = bit 16c+i of coefficient j.
 template <int P, int SIZE>
-ZX_HD void to_planes(const typename Ints<P>::U (&u)[SIZE], typename PWord<SIZE>::T (&pl)[P])
+ZX_HD void plane_chunk(const typename Ints<P>::U (&u)[SIZE], int c,
+                       typename PWord<SIZE>::T (&out)[16])
 {
-  constexpr int G = SIZE > 32 ? 2 : 1;   // coefficient groups of 32
-  constexpr int H = P / 32;              // bit groups of 32
+  typedef typename PWord<SIZE>::T W;
+  constexpr int G = SIZE > 32 ? 2 : 1;
+  const int h = c >> 1;  // 32-bit group of the coefficients
   ZX_UNROLL
-  for (int h = 0; h < H; h++) {
+  for (int g = 0; g < G; g++) {
+    uint32_t a[32];
     ZX_UNROLL
-    for (int g = 0; g < G; g++) {
-      uint32_t t[32];
-      ZX_UNROLL
-      for (int r = 0; r < 32; r++) {
-        const int i = 32 * g + r;
-        t[r] = i < SIZE ? (uint32_t)((uint64_t)u[i < SIZE ? i : 0] >> (32 * h)) : 0u;
-      }
-      tr32(t);
-      ZX_UNROLL
-      for (int c = 0; c < 32; c++) {
-        typedef typename PWord<SIZE>::T W;
-        if (g == 0) pl[32 * h + c] = (W)t[c];
-        else pl[32 * h + c] |= (W)((uint64_t)t[c] << 32);
-      }
+    for (int r = 0; r < 32; r++) {
+      const int i = 32 * g + r;
+      a[r] = i < SIZE ? (uint32_t)((uint64_t)u[i < SIZE ? i : 0] >> (32 * h)) : 0u;
+    }
+    uint32_t o[16];
+    if (c & 1) tr32_half<true>(a, o);
+    else tr32_half<false>(a, o);
+    ZX_UNROLL
+    for (int i = 0; i < 16; i++) {
+      if (g == 0) out[i] = (W)o[i];
+      else out[i] |= (W)((uint64_t)o[i] << 32);
     }
   }
 }
 
+// ORs the decoded planes 16c..16c+15 (pl[i] bit j = bit 16c+i of coefficient
+// j) into the coefficients: a 16 x 32 bit-matrix transpose per 32 coefficients.
 template <int P, int SIZE>
-ZX_HD void from_planes(const typename PWord<SIZE>::T (&pl)[P], typename Ints<P>::U (&u)[SIZE])
+ZX_HD void chunk_to_coeffs(const typename PWord<SIZE>::T (&pl)[16], int c,
+                           typename Ints<P>::U (&u)[SIZE])
 {
   typedef typename Ints<P>::U U;
   constexpr int G = SIZE > 32 ? 2 : 1;
-  constexpr int H = P / 32;
   ZX_UNROLL
-  for (int i = 0; i < SIZE; i++) u[i] = 0;
-  ZX_UNROLL
-  for (int h = 0; h < H; h++) {
+  for (int g = 0; g < G; g++) {
+    uint32_t t[16];
     ZX_UNROLL
-    for (int g = 0; g < G; g++) {
-      uint32_t t[32];
-      ZX_UNROLL
-      for (int c = 0; c < 32; c++) t[c] = (uint32_t)((uint64_t)pl[32 * h + c] >> (32 * g));
-      tr32(t);
-      ZX_UNROLL
-      for (int r = 0; r < 32; r++) {
-        const int i = 32 * g + r;
-        if (i < SIZE) u[i < SIZE ? i : 0] |= (U)((uint64_t)t[r] << (32 * h));
-      }
+    for (int i = 0; i < 16; i++) t[i] = (uint32_t)((uint64_t)pl[i] >> (32 * g));
+    // rows 16..31 of a 32 x 32 transpose whose rows 0..15 are zero: after the
+    // stages, t[i] holds coefficient i in its low half and i + 16 in its high
+    tr_stage<8>(t);
+    tr_stage<4>(t);
+    tr_stage<2>(t);
+    tr_stage<1>(t);
+    ZX_UNROLL
+    for (int i = 0; i < 16; i++) {
+      const int j0 = 32 * g + i, j1 = 32 * g + i + 16;
+      if (j0 < SIZE) u[j0 < SIZE ? j0 : 0] |= (U)(t[i] & 0xffffu) << (16 * c);
+      if (j1 < SIZE) u[j1 < SIZE ? j1 : 0] |= (U)(t[i] >> 16) << (16 * c);
     }
   }
 }
 
 // ---------------------------------------------------------------- bit sinks
+
+// Append the low n bits (no bits set above them) of a 128-bit value (lo, hi),
+// n <= 128, to a 64-bit buffer holding `bits` < 64 bits; returns the number
+// of completed words (0..2) in w0, w1 and leaves the remainder in buf.
+ZX_HD uint32_t append128(uint64_t &buf, uint32_t &bits, uint64_t lo, uint64_t hi, uint32_t n,
+                         uint64_t &w0, uint64_t &w1)
+{
+  const uint32_t b = bits;
+  w0 = buf | (lo << b);
+  const uint64_t lo_sp = (lo >> 1) >> (63 - b);  // lo >> (64 - b), 0 when b == 0
+  w1 = lo_sp | (hi << b);
+  const uint64_t hi_sp = (hi >> 1) >> (63 - b);
+  const uint32_t t = b + n;
+  const uint32_t full = t >> 6;
+  buf = full == 0 ? w0 : full == 1 ? w1 : hi_sp;
+  bits = t & 63;
+  return full;
+}
 
 // Writes a block's bits into a word array.  Words the block shares with its
 // neighbours (the first, if the block starts mid-word, and the last partial
@@ -384,23 +404,18 @@ struct WordSink {
     shared_first = false;
     idx++;
   }
-  // append the n <= 64 low bits of v (no bits set above them)
-  ZX_HD void put(uint64_t v, uint32_t n)
+  ZX_HD void put128(uint64_t lo, uint64_t hi, uint32_t n)
   {
-    buf |= v << bits;
-    uint32_t t = bits + n;
-    if (t >= 64) {
-      emit(buf);
-      buf = (v >> 1) >> (63 - bits);
-      bits = t - 64;
-    } else {
-      bits = t;
-    }
+    uint64_t w0, w1;
+    const uint32_t full = append128(buf, bits, lo, hi, n, w0, w1);
+    if (full >= 1) emit(w0);
+    if (full >= 2) emit(w1);
   }
+  ZX_HD void put(uint64_t v, uint32_t n) { put128(v, 0, n); }
   ZX_HD void zeros(uint32_t n)
   {
-    while (n > 64) { put(0, 64); n -= 64; }
-    put(0, n);
+    while (n > 128) { put128(0, 0, 128); n -= 128; }
+    put128(0, 0, n);
   }
   ZX_HD void finish()
   {
@@ -408,39 +423,119 @@ struct WordSink {
   }
 };
 
+// Writes a block's bits into a word array in LDS (or any memory one thread
+// owns), branch free: every append stores its two candidate words and the
+// index advances by the number completed.  Word i lives at
+// base[i + (i >> skew)] (a pad word per 2^skew words keeps lanes on separate
+// banks).  With SHARED, stores are atomic ORs into zeroed words, for blocks
+// that share boundary words with their neighbours.
+template <bool SHARED>
+struct LdsSink {
+  uint64_t *base;
+  uint32_t idx;
+  uint32_t skew;
+  uint64_t buf;
+  uint32_t bits;
+
+  ZX_HD void init(uint64_t *b, uint32_t sk, uint32_t bitpos)
+  {
+    base = b;
+    skew = sk;
+    idx = bitpos >> 6;
+    bits = bitpos & 63;
+    buf = 0;
+  }
+  ZX_HD uint64_t *at(uint32_t i) const { return base + i + (i >> skew); }
+  ZX_HD void store(uint32_t i, uint64_t v)
+  {
+    if constexpr (SHARED) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      atomicOr((unsigned long long *)at(i), (unsigned long long)v);
+#else
+      *at(i) |= v;
+#endif
+    } else {
+      *at(i) = v;
+    }
+  }
+  ZX_HD void put128(uint64_t lo, uint64_t hi, uint32_t n)
+  {
+    uint64_t w0, w1;
+    const uint32_t full = append128(buf, bits, lo, hi, n, w0, w1);
+    if constexpr (SHARED) {
+      // OR only completed words; a partial word is carried in buf
+      if (full >= 1) store(idx, w0);
+      if (full >= 2) store(idx + 1, w1);
+    } else {
+      // candidates: a word stored before it is complete is rewritten later
+      store(idx, w0);
+      store(idx + 1, w1);
+    }
+    idx += full;
+  }
+  ZX_HD void put(uint64_t v, uint32_t n) { put128(v, 0, n); }
+  ZX_HD void zeros(uint32_t n)
+  {
+    if (n == 0) return;  // no candidate stores once the block's words are done
+    while (n > 128) { put128(0, 0, 128); n -= 128; }
+    put128(0, 0, n);
+  }
+  ZX_HD void finish()
+  {
+    if (bits) store(idx, buf);
+  }
+};
+
 // Counts bits only (the length pass of variable-rate compression).
 struct CountSink {
   uint64_t n;
   ZX_HD void put(uint64_t, uint32_t k) { n += k; }
+  ZX_HD void put128(uint64_t, uint64_t, uint32_t k) { n += k; }
   ZX_HD void zeros(uint32_t k) { n += k; }
   ZX_HD void finish() {}
 };
 
-// Reads a block's bits; words past `nwords` read as zero (bitstream/core.rs:151-157).
-struct WordSource {
-  const uint64_t *words;
-  uint64_t nwords;
-  uint64_t idx;  // index of `nxt`
+// Word memories for sources: global words with a bound (past it reads zero,
+// bitstream/core.rs:151-157), or a staged copy addressed with a skew.
+struct GlobalWords {
+  const uint64_t *w;
+  uint64_t n;
+  ZX_HD uint64_t operator()(uint64_t i) const { return i < n ? w[i] : 0ull; }
+};
+struct StagedWords {
+  const uint64_t *w;
+  uint32_t skew;
+  ZX_HD uint64_t operator()(uint64_t i) const
+  {
+    const uint32_t j = (uint32_t)i;
+    return w[j + (j >> skew)];
+  }
+};
+
+// Reads a block's bits: a buffer of `bits` bits plus the next word.  `idx` is
+// the index of that next word, which equals the reference cursor's word_pos.
+template <class Mem>
+struct Source {
+  Mem mem;
+  uint64_t idx;
   uint64_t buf;
   uint64_t nxt;
   uint32_t bits;
 
-  ZX_HD uint64_t load(uint64_t i) const { return i < nwords ? words[i] : 0ull; }
-  ZX_HD void init(const uint64_t *w, uint64_t n, uint64_t bitpos)
+  ZX_HD void init(const Mem &m, uint64_t bitpos)
   {
-    words = w;
-    nwords = n;
+    mem = m;
     idx = bitpos >> 6;
-    uint32_t s = (uint32_t)(bitpos & 63);
+    const uint32_t s = (uint32_t)(bitpos & 63);
     if (s) {
-      buf = load(idx) >> s;
+      buf = mem(idx) >> s;
       bits = 64 - s;
       idx++;
     } else {
       buf = 0;
       bits = 0;
     }
-    nxt = load(idx);
+    nxt = mem(idx);
   }
   ZX_HD uint64_t peek() const { return buf | (nxt << bits); }
   ZX_HD void consume(uint32_t n)
@@ -452,7 +547,7 @@ struct WordSource {
       bits = bits + 64 - n;
       buf = (nxt >> 1) >> (63 - bits);
       idx++;
-      nxt = load(idx);
+      nxt = mem(idx);
     }
   }
   ZX_HD uint64_t read(uint32_t n)
@@ -461,11 +556,17 @@ struct WordSource {
     consume(n);
     return v & lowmask(n);
   }
+  ZX_HD uint64_t pos() const { return idx * 64 - bits; }
 };
+
+typedef Source<GlobalWords> WordSource;
 
 // ---------------------------------------------------------------- plane coder
 
-// encode_ints + encode_planes (bitplane.rs:509-610).  Returns bits written.
+// encode_ints + encode_planes (bitplane.rs:509-610): each plane's code (the
+// first n bits verbatim, then the group tests) is built in registers and
+// appended once, cut to the budget; planes are transposed 16 at a time, only
+// as far down as some lane of the wave still codes.  Returns bits written.
 template <int P, int SIZE, class Sink>
 ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
                            const typename Ints<P>::U (&u)[SIZE])
@@ -478,8 +579,6 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   ZX_UNROLL
   for (int i = 0; i < SIZE; i++) acc |= u[i];
   const uint32_t top = P == 32 ? 32u - clz32((uint32_t)acc) : 64u - clz64((uint64_t)acc);
-  PW pl[P];
-  to_planes<P, SIZE>(u, pl);
 
   uint32_t bits = budget;
   uint32_t empty = (uint32_t)P - (top > kmin ? top : kmin);
@@ -488,32 +587,69 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   bits -= empty;
   const uint32_t kstart = (uint32_t)P - empty;  // planes below kstart remain
   uint32_t n = 0;
+  // planes of the current 16-plane chunk; a chunk no lane of the wave codes
+  // is not transposed (its stale contents are never read)
+  PW ch[16];
   ZX_UNROLL
   for (int k = P - 1; k >= 0; k--) {
+    if ((k & 15) == 15) {
+      const int c = k >> 4;
+      const bool need = bits != 0 && kstart > (uint32_t)(16 * c) && kmin <= (uint32_t)k;
+      if (wave_any(need)) plane_chunk<P, SIZE>(u, c, ch);
+    }
     if (bits != 0 && (uint32_t)k < kstart && (uint32_t)k >= kmin) {
-      const uint64_t x = (uint64_t)pl[k];
-      // step 1: the first n coefficients are significant; their bits verbatim
-      uint32_t m = umin(n, bits);
-      bits -= m;
-      w.put(x & lowmask(m), m);
-      // step 2: group tests on the rest
-      while (bits != 0 && n < (uint32_t)SIZE) {
-        uint64_t rest = x >> n;
+      const uint64_t x = (uint64_t)ch[k & 15];
+      uint64_t clo, chi = 0;
+      uint32_t len, nn = n;
+      if (n >= (uint32_t)SIZE) {
+        // every coefficient is significant: the plane verbatim
+        clo = x;
+        len = SIZE;
+      } else {
+        const uint64_t rest = x >> n;
+        clo = x & lowmask(n);
         if (rest == 0) {
-          w.put(0, 1);
-          bits -= 1;
-          break;
+          len = n + 1;  // one negative group test
+        } else {
+          // E holds one-bit j (input position q_j) at q_j + j; the group
+          // tests then read 1, E << 1 (the one-bits) and E << 2 less its top
+          // (the positive tests between them)
+          uint64_t elo = 0, ehi = 0, r = rest;
+          uint32_t j = 0, q = 0;
+          while (r) {
+            q = ctz64(r);
+            const uint32_t b = q + j;
+            const uint64_t t = 1ull << (b & 63);
+            elo |= b < 64 ? t : 0ull;
+            ehi |= b < 64 ? 0ull : t;
+            r &= r - 1;
+            j++;
+          }
+          const uint32_t tb = q + j - 1;
+          const uint64_t tm = 1ull << (tb & 63);
+          const uint64_t plo = tb < 64 ? elo & ~tm : elo;
+          const uint64_t phi = tb < 64 ? ehi : ehi & ~tm;
+          uint64_t glo = 1ull | (elo << 1) | (plo << 2);
+          uint64_t ghi = (ehi << 1) | (elo >> 63) | (phi << 2) | (plo >> 62);
+          uint32_t glen = tb + 3;
+          if (n + q == (uint32_t)SIZE - 1) {
+            // the one-bit at the last position is implied, with no test after
+            const uint32_t d = tb + 1;
+            const uint64_t dm = 1ull << (d & 63);
+            glo = d < 64 ? glo & ~dm : glo;
+            ghi = d < 64 ? ghi : ghi & ~dm;
+            glen = tb + 1;
+          }
+          clo |= glo << n;
+          chi = (ghi << n) | ((glo >> 1) >> (63 - n));
+          len = n + glen;
+          nn = n + q + 1;
         }
-        uint32_t p = n + ctz64(rest);
-        uint32_t zeros = p - n;
-        bool last = p == (uint32_t)SIZE - 1;
-        uint32_t len = zeros + 2 - (last ? 1u : 0u);
-        uint32_t take = umin(len, bits);  // take <= 64 for SIZE <= 64
-        uint64_t code = last ? 1ull : (1ull | (2ull << zeros));
-        w.put(code & lowmask(take), take);
-        bits -= take;
-        n = p + 1;
       }
+      const uint32_t take = umin(len, bits);
+      w.put128(clo & lowmask(take), take > 64 ? chi & lowmask(take - 64) : 0ull, take);
+      bits -= take;
+      n = nn;
     }
   }
   return budget - bits;
@@ -532,9 +668,12 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
   const bool constrained = with_maxbits(maxbits, maxprec, SIZE);
   const uint32_t budget = constrained ? maxbits : 0xffffffffu;
   const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
-  PW pl[P];
+  // planes of the current 16-plane chunk, folded into u as each completes
+  PW pl[16];
   ZX_UNROLL
-  for (int k = 0; k < P; k++) pl[k] = 0;
+  for (int i = 0; i < 16; i++) pl[i] = 0;
+  ZX_UNROLL
+  for (int i = 0; i < SIZE; i++) u[i] = 0;
   uint32_t bits = budget, n = 0, m = 0, k = P;
   bool act = true;
   ZX_UNROLL
@@ -562,22 +701,25 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
             }
             uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
             uint32_t zeros = ctz64(ahead >> 1);
-            if (zeros < limit) {
-              r.consume(zeros + 2);
-              bits -= zeros + 1;
-              n += zeros;
-            } else {
-              r.consume(limit + 1);
-              bits -= limit;
-              n += limit;
-            }
+            uint32_t adv = zeros < limit ? zeros : limit;
+            r.consume(adv + 1 + (zeros < limit ? 1u : 0u));
+            bits -= adv + (zeros < limit ? 1u : 0u);
+            n += adv;
             x |= 1ull << n;
             n += 1;
             m = n;
           }
-          pl[kk] = (PW)x;
+          pl[kk & 15] = (PW)x;
         }
       }
+    }
+    if ((kk & 15) == 0) {
+      PW any = 0;
+      ZX_UNROLL
+      for (int i = 0; i < 16; i++) any |= pl[i];
+      if (wave_any(any != 0)) chunk_to_coeffs<P, SIZE>(pl, kk >> 4, u);
+      ZX_UNROLL
+      for (int i = 0; i < 16; i++) pl[i] = 0;
     }
   }
   if (act && bits != 0) {  // ran through plane 0 with budget left (kmin == 0)
@@ -585,12 +727,6 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
     k -= 1;
   }
   const bool zero = n == 0;
-  if (zero) {
-    ZX_UNROLL
-    for (int i = 0; i < SIZE; i++) u[i] = 0;
-  } else {
-    from_planes<P, SIZE>(pl, u);
-  }
   if (round_last) {
     // inv_round (decode/core.rs:62-82) with round_state (bitplane.rs:661-667)
     uint32_t rm = constrained ? m : 0u;
@@ -656,9 +792,10 @@ ZX_HD uint32_t decode_int_block(Src &r, typename Ints<P>::I (&q)[Blk<D>::SIZE], 
 
 // ---------------------------------------------------------------- float helpers
 
-// frexp exponent of the largest non-NaN magnitude (encode/core.rs:280-307)
+// frexp exponent of the largest non-NaN magnitude (encode/core.rs:280-307).
+// `special` is set when the block holds an infinity or a NaN.
 template <class S, int SIZE>
-ZX_HD int exponent_block(const typename Ints<ST<S>::P>::U (&b)[SIZE])
+ZX_HD int exponent_block(const typename Ints<ST<S>::P>::U (&b)[SIZE], bool &special)
 {
   typedef typename Ints<ST<S>::P>::U U;
   constexpr int P = ST<S>::P, EB = ST<S>::EBITS, MANT = ST<S>::MANT;
@@ -667,15 +804,36 @@ ZX_HD int exponent_block(const typename Ints<ST<S>::P>::U (&b)[SIZE])
   U mx = 0;
   ZX_UNROLL
   for (int i = 0; i < SIZE; i++) {
-    U a = b[i] & absmask;
-    a = a > inf ? (U)0 : a;
+    const U a = b[i] & absmask;
     mx = a > mx ? a : mx;
+  }
+  special = mx >= inf;
+  if (special) {
+    // NaNs take no part in the maximum
+    mx = 0;
+    ZX_UNROLL
+    for (int i = 0; i < SIZE; i++) {
+      U a = b[i] & absmask;
+      a = a > inf ? (U)0 : a;
+      mx = a > mx ? a : mx;
+    }
   }
   if (mx == 0) return -ST<S>::EBIAS;
   const int biased = (int)(mx >> MANT);
   if (biased == (1 << EB) - 1) return 0;
   return biased - (ST<S>::EBIAS - 1);
 }
+
+template <class S, int SIZE>
+ZX_HD int exponent_block(const typename Ints<ST<S>::P>::U (&b)[SIZE])
+{
+  bool special;
+  return exponent_block<S, SIZE>(b, special);
+}
+
+// 2^k as an exact normal float / double
+ZX_HD float pow2f(int k) { return u2f((uint32_t)(k + 127) << 23); }
+ZX_HD double pow2d(int k) { return u2d((uint64_t)(k + 1023) << 52); }
 
 // precision_f (encode/core.rs:231-244)
 ZX_HD uint32_t precision_f(int emax, uint32_t maxprec, int32_t minexp, int dims, int tight)
@@ -814,16 +972,33 @@ ZX_HD uint32_t encode_lossy(Sink &w, const typename Ints<ST<S>::P>::U (&b)[Blk<D
     for (int i = 0; i < SIZE; i++) q[i] = (I)b[i];
     return encode_int_block<P, D>(w, q, c.minbits, c.maxbits, c.maxprec, c.rounding);
   } else {
-    const int emax = exponent_block<S, SIZE>(b);
+    bool special;
+    const int emax = exponent_block<S, SIZE>(b, special);
     const uint32_t prec = precision_f(emax, c.maxprec, c.minexp, D, c.tight);
     const uint32_t e = prec ? (uint32_t)(emax + ST<S>::EBIAS) : 0u;
     if (e) {
       const uint32_t hb = ST<S>::EBITS + 1;
       w.put(2ull * e + 1, hb);
-      ZX_UNROLL
-      for (int i = 0; i < SIZE; i++) {
-        bool ex;
-        q[i] = cast_one<S>(b[i], emax, ex);
+      // Without infinities or NaNs, and above the range where a subnormal
+      // value can scale to one or more, x * 2^(P-2-emax) is exact wherever
+      // it is at least one, so the hardware product and truncation give
+      // trunc(x * 2^(P-2-emax)) exactly.
+      if (!special && emax >= ST<S>::MINCAST + 1) {
+        if constexpr (P == 32) {
+          const float sc = pow2f(30 - emax);
+          ZX_UNROLL
+          for (int i = 0; i < SIZE; i++) q[i] = (I)(u2f((uint32_t)b[i]) * sc);
+        } else {
+          const double sc = pow2d(62 - emax);
+          ZX_UNROLL
+          for (int i = 0; i < SIZE; i++) q[i] = (I)(u2d((uint64_t)b[i]) * sc);
+        }
+      } else {
+        ZX_UNROLL
+        for (int i = 0; i < SIZE; i++) {
+          bool ex;
+          q[i] = cast_one<S>(b[i], emax, ex);
+        }
       }
       return hb + encode_int_block<P, D>(w, q, sat_sub(c.minbits, hb), sat_sub(c.maxbits, hb), prec,
                                          c.rounding);

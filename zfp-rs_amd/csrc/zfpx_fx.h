@@ -1,0 +1,172 @@
+// zfpx_fx.h -- the fixed-rate LDS path, written as per-thread phases that the
+// kernels run between workgroup barriers and the CPU test build runs one
+// thread after another.
+//
+// A workgroup codes T consecutive blocks.  At a fixed rate its blocks occupy
+// one contiguous bit range of the stream (block b at start + b * fixed, the
+// reference's per-chunk seek, src/compress/pipeline.rs and
+// src/decompress/pipeline.rs fixed-rate paths), so the workgroup moves its
+// whole word range between HBM and LDS with coalesced accesses while every
+// thread codes its own block against LDS.
+//
+// Encode layouts:
+//  - OWN: blocks are whole words and block 0 starts on a word, so thread t
+//    owns words [t*W, t*W + W) outright, kept at LDS stride W + 1 (the extra
+//    word is the thread's private spill slot for branch-free candidate
+//    stores, and it staggers the lanes across LDS banks);
+//  - shared: blocks straddle words; the range is kept contiguous with a pad
+//    word every 2^skew words, zeroed first, and neighbours merge the words
+//    they share with LDS atomic ORs.  The range's first and last words may be
+//    shared with the neighbouring workgroups and are merged into HBM with
+//    atomic ORs (the host zeroes the stream's range beforehand); all other
+//    words are stored whole.
+// Decode stages the range (plus one word the reader looks ahead into) in the
+// contiguous layout; words at or past the stream's capacity read as zero.
+#pragma once
+
+#include "zfpx_field.h"
+
+namespace zx {
+
+struct FxPlan {
+  uint32_t T;          // blocks (threads) per workgroup
+  uint32_t skew;       // contiguous layouts: word i at LDS slot i + (i >> skew)
+  bool own;            // encode layout OWN (see above)
+  uint32_t lds_words;  // LDS words per workgroup
+};
+
+ZX_HD uint32_t fx_log2(uint32_t v)
+{
+  uint32_t s = 0;
+  while ((2u << s) <= v) s++;
+  return s;
+}
+
+// Plan of the LDS path for blocks of `fixed` bits from bit `start`; false when
+// blocks are too large for it.  `T` overrides the workgroup size when nonzero.
+ZX_HD bool fx_plan(uint32_t fixed, uint64_t start, bool decode, uint32_t T, FxPlan &p)
+{
+  if (fixed == 0 || fixed > 64 * 64) return false;
+  const uint32_t W = (fixed + 63) / 64;
+  p.T = T ? T : W <= 16 ? 256u : W <= 32 ? 128u : 64u;
+  p.own = !decode && fixed % 64 == 0 && start % 64 == 0;
+  const uint32_t s = fx_log2(W);
+  p.skew = p.own ? 31u : s < 3 ? 3u : s;
+  if (p.own) {
+    p.lds_words = p.T * (W + 1);
+  } else {
+    const uint32_t words = (uint32_t)(((uint64_t)p.T * fixed + 63) / 64) + 3;
+    p.lds_words = words + (words >> p.skew) + 1;
+  }
+  return true;
+}
+
+// One workgroup's share of the stream.
+struct FxGroup {
+  uint64_t b0;  // first block
+  uint32_t nb;  // blocks
+  uint64_t w0;  // first word touched
+  uint32_t r0;  // bit of block b0 within word w0
+  uint32_t nw;  // words touched
+  bool first_shared, last_shared;
+};
+
+ZX_HD FxGroup fx_group(uint64_t g, uint32_t T, uint64_t nblocks, uint64_t start, uint32_t fixed)
+{
+  FxGroup q;
+  q.b0 = g * T;
+  q.nb = nblocks - q.b0 < T ? (uint32_t)(nblocks - q.b0) : T;
+  const uint64_t bit0 = start + q.b0 * fixed;
+  const uint64_t bit_end = bit0 + (uint64_t)q.nb * fixed;
+  q.w0 = bit0 >> 6;
+  q.r0 = (uint32_t)(bit0 & 63);
+  q.nw = (uint32_t)(((bit_end + 63) >> 6) - q.w0);
+  q.first_shared = q.r0 != 0;
+  q.last_shared = (bit_end & 63) != 0;
+  return q;
+}
+
+ZX_HD uint32_t fx_slot(uint32_t i, uint32_t skew) { return i + (i >> skew); }
+
+// Words a block loaded, as the reference's cursor counts them (one past the
+// highest): those it consumed bits of, plus the word a skip to min_bits lands
+// inside (decompress.rs:57-74, bitstream/core.rs:265-284).  Positions are
+// relative to the reader's word 0.
+template <class Src>
+ZX_HD uint64_t loaded_words(const Src &r, uint64_t pos, uint32_t minbits)
+{
+  const uint64_t used = r.pos() - pos;
+  uint64_t loaded = r.idx;
+  if (used < minbits) {
+    const uint64_t target = pos + minbits;
+    if (target % 64) loaded = loaded > target / 64 + 1 ? loaded : target / 64 + 1;
+  }
+  return loaded;
+}
+
+// ------------------------------------------------------------------ encode
+
+// phase 1 (shared layout only): zero word slot i
+ZX_HD void fx_enc_clear(uint64_t *lds, const FxPlan &p, uint32_t i) { lds[fx_slot(i, p.skew)] = 0; }
+
+// phase 2: thread t encodes its block's values v
+template <class S, int D, bool REV, bool OWN>
+ZX_HD void fx_enc_block(uint64_t *lds, const FxPlan &p, const FxGroup &q, uint32_t t, uint32_t fixed,
+                        const typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE], const Cfg &c)
+{
+  if constexpr (OWN) {
+    LdsSink<false> w;
+    w.init(lds + (size_t)t * (fixed / 64 + 1), 31, 0);
+    encode_values<S, D, REV>(w, v, c);
+    w.finish();
+  } else {
+    LdsSink<true> w;
+    w.init(lds, p.skew, q.r0 + t * fixed);
+    encode_values<S, D, REV>(w, v, c);
+    w.finish();
+  }
+}
+
+// phase 3: word i of the group to the stream; false past the capacity
+template <bool OWN>
+ZX_HD bool fx_enc_store(uint64_t *words, uint64_t nwords, const uint64_t *lds, const FxPlan &p,
+                        const FxGroup &q, uint32_t i, uint32_t fixed)
+{
+  const uint64_t gw = q.w0 + i;
+  if (gw >= nwords) return false;  // the reference drops writes past the end
+  if constexpr (OWN) {
+    const uint32_t W = fixed / 64;
+    words[gw] = lds[i + i / W];
+  } else {
+    const uint64_t v = lds[fx_slot(i, p.skew)];
+    if ((i == 0 && q.first_shared) || (i == q.nw - 1 && q.last_shared)) WordSink::merge(words + gw, v);
+    else words[gw] = v;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------------ decode
+
+// phase 1: stage word i (i < nw + 1)
+ZX_HD void fx_dec_stage(uint64_t *lds, const uint64_t *words, uint64_t nwords, const FxPlan &p,
+                        const FxGroup &q, uint32_t i)
+{
+  const uint64_t gw = q.w0 + i;
+  lds[fx_slot(i, p.skew)] = gw < nwords ? words[gw] : 0ull;
+}
+
+// phase 2: thread t decodes its block into v; returns one past the highest
+// stream word the reference's reader would have loaded for it
+template <class S, int D, bool REV>
+ZX_HD uint64_t fx_dec_block(const uint64_t *lds, const FxPlan &p, const FxGroup &q, uint32_t t,
+                            uint32_t fixed, typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE],
+                            const Cfg &c)
+{
+  const uint64_t pos = q.r0 + (uint64_t)t * fixed;
+  Source<StagedWords> r;
+  r.init(StagedWords{lds, p.skew}, pos);
+  decode_values<S, D, REV>(r, v, c);
+  return q.w0 + loaded_words(r, pos, c.minbits);
+}
+
+}  // namespace zx

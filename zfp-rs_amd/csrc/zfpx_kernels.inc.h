@@ -4,15 +4,24 @@
 // One thread codes one block.  Blocks are placed in the stream either at a
 // fixed stride (fixed-rate: block b starts at start + b * bits, the
 // reference's per-chunk seek, decompress.rs:205-330) or at offsets from an
-// exclusive scan of per-block lengths (variable-rate).  Stream words a block
-// owns outright are stored whole; the few words shared by neighbouring blocks
-// are merged with an atomic OR into words the host zeroed first.
+// exclusive scan of per-block lengths (variable-rate).
+//
+// Fixed-rate blocks of up to 64 words go through LDS: a workgroup's slice of
+// the stream is one contiguous word range, read in (decode) or written out
+// (encode) with coalesced accesses, while each thread works on its own words
+// in LDS.  When blocks are whole words and the stream starts on a word, each
+// thread owns its words outright (stride W + 1 with a private pad word);
+// otherwise neighbours share boundary words and merge them with LDS atomic
+// ORs, and the workgroup's two end words are merged into global memory.
+// Other layouts use global memory directly (zfpx_block.h WordSink /
+// WordSource), with the host zeroing the word range first.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "zfpx_api.h"
 #include "zfpx_field.h"
+#include "zfpx_fx.h"
 
 namespace zx {
 
@@ -82,6 +91,8 @@ __device__ __forceinline__ void store_block(const FieldDesc &f, uint64_t b, bool
   else scatter<S, D>(f, p, v);
 }
 
+// ------------------------------------------------------------------ global path
+
 // Encode every block; COUNT writes per-block bit lengths instead of bits.
 template <class S, int D, bool REV, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_encode(EncodeArgs a)
@@ -110,18 +121,11 @@ __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
   if (b >= a.field.nblocks) return;
   const uint64_t pos = a.offsets ? a.offsets[b] : a.start + b * (uint64_t)a.fixed;
   WordSource r;
-  r.init(a.words, a.nwords, pos);
+  r.init(GlobalWords{a.words, a.nwords}, pos);
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
   decode_values<S, D, REV>(r, v, a.cfg);
-  // Words this block loaded, as the reference's cursor counts them: those it
-  // consumed bits of, plus the word a skip to min_bits lands inside.
-  const uint64_t used = r.idx * 64 - r.bits - pos;
-  uint64_t loaded = r.idx;
-  if (used < a.cfg.minbits) {
-    const uint64_t target = pos + a.cfg.minbits;
-    if (target % 64) loaded = loaded > target / 64 + 1 ? loaded : target / 64 + 1;
-  }
-  atomicMax((unsigned long long *)a.maxword, (unsigned long long)loaded);
+  // blocks are contiguous, so the last one loads the highest word
+  if (b == a.field.nblocks - 1) *a.maxword = loaded_words(r, pos, a.cfg.minbits);
   store_block<S, D>(a.field, b, a.vec, v);
 }
 
@@ -135,21 +139,72 @@ __global__ __launch_bounds__(64) void k_walk(DecodeArgs a, uint32_t *lengths)
   uint64_t pos = a.start;
   for (uint64_t b = 0; b < a.field.nblocks; b++) {
     WordSource r;
-    r.init(a.words, a.nwords, pos);
+    r.init(GlobalWords{a.words, a.nwords}, pos);
     typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
     decode_values<S, D, REV>(r, v, a.cfg);
-    uint64_t used = r.idx * 64 - r.bits - pos;
+    uint64_t used = r.pos() - pos;
     if (used < a.cfg.minbits) used = a.cfg.minbits;
     lengths[b] = (uint32_t)used;
     pos += used;
   }
 }
 
+// ------------------------------------------------------------------ LDS path
+
+// Fixed-rate encode through LDS (zfpx_fx.h): one workgroup per group of
+// blockDim.x blocks.
+template <class S, int D, bool REV, bool OWN>
+__global__ __launch_bounds__(kThreads) void k_encode_fx(EncodeArgs a, FxPlan p)
+{
+  extern __shared__ uint64_t lds[];
+  const uint32_t t = threadIdx.x;
+  const FxGroup q = fx_group(blockIdx.x, p.T, a.field.nblocks, a.start, a.fixed);
+  if constexpr (!OWN) {
+    for (uint32_t i = t; i < q.nw; i += p.T) fx_enc_clear(lds, p, i);
+    __syncthreads();
+  }
+  if (t < q.nb) {
+    typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+    load_block<S, D>(a.field, q.b0 + t, a.vec, v);
+    fx_enc_block<S, D, REV, OWN>(lds, p, q, t, a.fixed, v, a.cfg);
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < q.nw; i += p.T)
+    if (!fx_enc_store<OWN>(a.words, a.nwords, lds, p, q, i, a.fixed)) break;
+}
+
+// Fixed-rate decode through LDS: stage the group's words, then decode.
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(kThreads) void k_decode_fx(DecodeArgs a, FxPlan p)
+{
+  extern __shared__ uint64_t lds[];
+  const uint32_t t = threadIdx.x;
+  const FxGroup q = fx_group(blockIdx.x, p.T, a.field.nblocks, a.start, a.fixed);
+  for (uint32_t i = t; i < q.nw + 1; i += p.T) fx_dec_stage(lds, a.words, a.nwords, p, q, i);
+  __syncthreads();
+  if (t >= q.nb) return;
+  typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+  const uint64_t loaded = fx_dec_block<S, D, REV>(lds, p, q, t, a.fixed, v, a.cfg);
+  // blocks are contiguous, so the last one loads the highest word
+  if (q.b0 + t == a.field.nblocks - 1) *a.maxword = loaded;
+  store_block<S, D>(a.field, q.b0 + t, a.vec, v);
+}
+
+// ------------------------------------------------------------------ launchers
+
 template <class S, int D, bool REV>
 static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
 {
+  if (a.field.nblocks == 0) return hipSuccess;
+  FxPlan p;
+  if (!a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, 0, p)) {
+    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+    const size_t bytes = (size_t)p.lds_words * 8;
+    if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
+    else hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
+    return hipGetLastError();
+  }
   const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
-  if (grid == 0) return hipSuccess;
   if (a.lengths) hipLaunchKernelGGL((k_encode<S, D, REV, true>), dim3(grid), dim3(kThreads), 0, s, a);
   else hipLaunchKernelGGL((k_encode<S, D, REV, false>), dim3(grid), dim3(kThreads), 0, s, a);
   return hipGetLastError();
@@ -162,8 +217,14 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
     hipLaunchKernelGGL((k_walk<S, D, REV>), dim3(1), dim3(64), 0, s, a, walk_lengths);
     return hipGetLastError();
   }
+  if (a.field.nblocks == 0) return hipSuccess;
+  FxPlan p;
+  if (!a.offsets && fx_plan(a.fixed, a.start, true, 0, p)) {
+    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+    hipLaunchKernelGGL((k_decode_fx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+    return hipGetLastError();
+  }
   const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
-  if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL((k_decode<S, D, REV>), dim3(grid), dim3(kThreads), 0, s, a);
   return hipGetLastError();
 }
