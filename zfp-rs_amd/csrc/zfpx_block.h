@@ -512,14 +512,16 @@ struct StagedWords {
   }
 };
 
-// Reads a block's bits: a buffer of `bits` bits plus the next word.  `idx` is
-// the index of that next word, which equals the reference cursor's word_pos.
+// Reads a block's bits: a buffer of `bits` bits plus the next two words (the
+// second is loaded a word ahead of use, so a load's latency overlaps the
+// decoding of the word before).  `idx` is the index of the next word, which
+// equals the reference cursor's word_pos.
 template <class Mem>
 struct Source {
   Mem mem;
   uint64_t idx;
   uint64_t buf;
-  uint64_t nxt;
+  uint64_t nxt, nxt2;
   uint32_t bits;
 
   ZX_HD void init(const Mem &m, uint64_t bitpos)
@@ -536,6 +538,7 @@ struct Source {
       bits = 0;
     }
     nxt = mem(idx);
+    nxt2 = mem(idx + 1);
   }
   ZX_HD uint64_t peek() const { return buf | (nxt << bits); }
   ZX_HD void consume(uint32_t n)
@@ -547,7 +550,8 @@ struct Source {
       bits = bits + 64 - n;
       buf = (nxt >> 1) >> (63 - bits);
       idx++;
-      nxt = mem(idx);
+      nxt = nxt2;
+      nxt2 = mem(idx + 1);
     }
   }
   ZX_HD uint64_t read(uint32_t n)
@@ -563,10 +567,62 @@ typedef Source<GlobalWords> WordSource;
 
 // ---------------------------------------------------------------- plane coder
 
-// encode_ints + encode_planes (bitplane.rs:509-610): each plane's code (the
-// first n bits verbatim, then the group tests) is built in registers and
-// appended once, cut to the budget; planes are transposed 16 at a time, only
-// as far down as some lane of the wave still codes.  Returns bits written.
+// Code of one bit plane x of a block whose first n coefficients are already
+// significant (bitplane.rs:560-605): the n low bits verbatim, then the group
+// tests over the rest.  Returns the code's length; the code is (lo, hi) and
+// nn is the new n.
+template <int SIZE>
+ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, uint32_t &nn)
+{
+  nn = n;
+  chi = 0;
+  if (n >= (uint32_t)SIZE) {
+    // every coefficient is significant: the plane verbatim
+    clo = x;
+    return SIZE;
+  }
+  const uint64_t rest = x >> n;
+  clo = x & lowmask(n);
+  if (rest == 0) return n + 1;  // one negative group test
+  // E holds one-bit j (input position q_j) at q_j + j; the group tests then
+  // read 1, E << 1 (the one-bits) and E << 2 less its top (the positive
+  // tests between them)
+  uint64_t elo = 0, ehi = 0, r = rest;
+  uint32_t j = 0, q = 0;
+  while (r) {
+    q = ctz64(r);
+    const uint32_t b = q + j;
+    const uint64_t t = 1ull << (b & 63);
+    elo |= b < 64 ? t : 0ull;
+    ehi |= b < 64 ? 0ull : t;
+    r &= r - 1;
+    j++;
+  }
+  const uint32_t tb = q + j - 1;
+  const uint64_t tm = 1ull << (tb & 63);
+  const uint64_t plo = tb < 64 ? elo & ~tm : elo;
+  const uint64_t phi = tb < 64 ? ehi : ehi & ~tm;
+  uint64_t glo = 1ull | (elo << 1) | (plo << 2);
+  uint64_t ghi = (ehi << 1) | (elo >> 63) | (phi << 2) | (plo >> 62);
+  uint32_t glen = tb + 3;
+  if (n + q == (uint32_t)SIZE - 1) {
+    // the one-bit at the last position is implied, with no test after
+    const uint32_t d = tb + 1;
+    const uint64_t dm = 1ull << (d & 63);
+    glo = d < 64 ? glo & ~dm : glo;
+    ghi = d < 64 ? ghi : ghi & ~dm;
+    glen = tb + 1;
+  }
+  clo |= glo << n;
+  chi = (ghi << n) | ((glo >> 1) >> (63 - n));
+  nn = n + q + 1;
+  return n + glen;
+}
+
+// encode_ints + encode_planes (bitplane.rs:509-610): each plane's code is
+// built in registers and appended once, cut to the budget.  Planes are
+// transposed 16 at a time (a chunk), and a chunk no lane of the wave codes
+// is skipped.  Returns bits written.
 template <int P, int SIZE, class Sink>
 ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
                            const typename Ints<P>::U (&u)[SIZE])
@@ -587,69 +643,24 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   bits -= empty;
   const uint32_t kstart = (uint32_t)P - empty;  // planes below kstart remain
   uint32_t n = 0;
-  // planes of the current 16-plane chunk; a chunk no lane of the wave codes
-  // is not transposed (its stale contents are never read)
-  PW ch[16];
   ZX_UNROLL
-  for (int k = P - 1; k >= 0; k--) {
-    if ((k & 15) == 15) {
-      const int c = k >> 4;
-      const bool need = bits != 0 && kstart > (uint32_t)(16 * c) && kmin <= (uint32_t)k;
-      if (wave_any(need)) plane_chunk<P, SIZE>(u, c, ch);
-    }
-    if (bits != 0 && (uint32_t)k < kstart && (uint32_t)k >= kmin) {
-      const uint64_t x = (uint64_t)ch[k & 15];
-      uint64_t clo, chi = 0;
-      uint32_t len, nn = n;
-      if (n >= (uint32_t)SIZE) {
-        // every coefficient is significant: the plane verbatim
-        clo = x;
-        len = SIZE;
-      } else {
-        const uint64_t rest = x >> n;
-        clo = x & lowmask(n);
-        if (rest == 0) {
-          len = n + 1;  // one negative group test
-        } else {
-          // E holds one-bit j (input position q_j) at q_j + j; the group
-          // tests then read 1, E << 1 (the one-bits) and E << 2 less its top
-          // (the positive tests between them)
-          uint64_t elo = 0, ehi = 0, r = rest;
-          uint32_t j = 0, q = 0;
-          while (r) {
-            q = ctz64(r);
-            const uint32_t b = q + j;
-            const uint64_t t = 1ull << (b & 63);
-            elo |= b < 64 ? t : 0ull;
-            ehi |= b < 64 ? 0ull : t;
-            r &= r - 1;
-            j++;
-          }
-          const uint32_t tb = q + j - 1;
-          const uint64_t tm = 1ull << (tb & 63);
-          const uint64_t plo = tb < 64 ? elo & ~tm : elo;
-          const uint64_t phi = tb < 64 ? ehi : ehi & ~tm;
-          uint64_t glo = 1ull | (elo << 1) | (plo << 2);
-          uint64_t ghi = (ehi << 1) | (elo >> 63) | (phi << 2) | (plo >> 62);
-          uint32_t glen = tb + 3;
-          if (n + q == (uint32_t)SIZE - 1) {
-            // the one-bit at the last position is implied, with no test after
-            const uint32_t d = tb + 1;
-            const uint64_t dm = 1ull << (d & 63);
-            glo = d < 64 ? glo & ~dm : glo;
-            ghi = d < 64 ? ghi : ghi & ~dm;
-            glen = tb + 1;
-          }
-          clo |= glo << n;
-          chi = (ghi << n) | ((glo >> 1) >> (63 - n));
-          len = n + glen;
-          nn = n + q + 1;
-        }
+  for (int c = P / 16 - 1; c >= 0; c--) {
+    const bool need = bits != 0 && kstart > (uint32_t)(16 * c) && kmin <= (uint32_t)(16 * c + 15);
+    if (!wave_any(need)) continue;
+    PW ch[16];
+    plane_chunk<P, SIZE>(u, c, ch);
+    ZX_UNROLL
+    for (int i = 15; i >= 0; i--) {
+      const uint32_t k = (uint32_t)(16 * c + i);
+      if (bits != 0 && k < kstart && k >= kmin) {
+        uint64_t clo, chi;
+        uint32_t nn;
+        const uint32_t len = plane_code<SIZE>((uint64_t)ch[i], n, clo, chi, nn);
+        const uint32_t take = umin(len, bits);
+        w.put128(clo & lowmask(take), take > 64 ? chi & lowmask(take - 64) : 0ull, take);
+        bits -= take;
+        n = nn;
       }
-      const uint32_t take = umin(len, bits);
-      w.put128(clo & lowmask(take), take > 64 ? chi & lowmask(take - 64) : 0ull, take);
-      bits -= take;
-      n = nn;
     }
   }
   return budget - bits;
@@ -668,59 +679,57 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
   const bool constrained = with_maxbits(maxbits, maxprec, SIZE);
   const uint32_t budget = constrained ? maxbits : 0xffffffffu;
   const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
-  // planes of the current 16-plane chunk, folded into u as each completes
-  PW pl[16];
-  ZX_UNROLL
-  for (int i = 0; i < 16; i++) pl[i] = 0;
   ZX_UNROLL
   for (int i = 0; i < SIZE; i++) u[i] = 0;
   uint32_t bits = budget, n = 0, m = 0, k = P;
   bool act = true;
   ZX_UNROLL
-  for (int kk = P - 1; kk >= 0; kk--) {
-    if (act) {
-      if (bits == 0) {
-        act = false;
-      } else {
-        m = 0;
-        if (k <= kmin) {
-          k -= 1;
+  for (int c = P / 16 - 1; c >= 0; c--) {
+    if (!wave_any(act)) break;
+    // planes 16c..16c+15, folded into u once the chunk is decoded
+    PW pl[16];
+    ZX_UNROLL
+    for (int i = 15; i >= 0; i--) {
+      pl[i] = 0;
+      if (act) {
+        if (bits == 0) {
           act = false;
         } else {
-          k -= 1;  // == kk
-          m = umin(n, bits);
-          bits -= m;
-          uint64_t x = r.read(m);
-          while (bits != 0 && n < (uint32_t)SIZE) {
-            uint64_t ahead = r.peek();
-            bits -= 1;
-            if ((ahead & 1u) == 0) {
-              r.consume(1);
-              m = SIZE;
-              break;
+          m = 0;
+          k -= 1;  // == 16c + i
+          if (k < kmin) {
+            act = false;
+          } else {
+            m = umin(n, bits);
+            bits -= m;
+            uint64_t x = r.read(m);
+            while (bits != 0 && n < (uint32_t)SIZE) {
+              uint64_t ahead = r.peek();
+              bits -= 1;
+              if ((ahead & 1u) == 0) {
+                r.consume(1);
+                m = SIZE;
+                break;
+              }
+              uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
+              uint32_t zeros = ctz64(ahead >> 1);
+              uint32_t adv = zeros < limit ? zeros : limit;
+              r.consume(adv + 1 + (zeros < limit ? 1u : 0u));
+              bits -= adv + (zeros < limit ? 1u : 0u);
+              n += adv;
+              x |= 1ull << n;
+              n += 1;
+              m = n;
             }
-            uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
-            uint32_t zeros = ctz64(ahead >> 1);
-            uint32_t adv = zeros < limit ? zeros : limit;
-            r.consume(adv + 1 + (zeros < limit ? 1u : 0u));
-            bits -= adv + (zeros < limit ? 1u : 0u);
-            n += adv;
-            x |= 1ull << n;
-            n += 1;
-            m = n;
+            pl[i] = (PW)x;
           }
-          pl[kk & 15] = (PW)x;
         }
       }
     }
-    if ((kk & 15) == 0) {
-      PW any = 0;
-      ZX_UNROLL
-      for (int i = 0; i < 16; i++) any |= pl[i];
-      if (wave_any(any != 0)) chunk_to_coeffs<P, SIZE>(pl, kk >> 4, u);
-      ZX_UNROLL
-      for (int i = 0; i < 16; i++) pl[i] = 0;
-    }
+    PW any = 0;
+    ZX_UNROLL
+    for (int i = 0; i < 16; i++) any |= pl[i];
+    if (wave_any(any != 0)) chunk_to_coeffs<P, SIZE>(pl, c, u);
   }
   if (act && bits != 0) {  // ran through plane 0 with budget left (kmin == 0)
     m = 0;

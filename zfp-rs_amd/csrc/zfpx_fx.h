@@ -20,7 +20,7 @@
 //    shared with the neighbouring workgroups and are merged into HBM with
 //    atomic ORs (the host zeroes the stream's range beforehand); all other
 //    words are stored whole.
-// Decode stages the range (plus one word the reader looks ahead into) in the
+// Decode stages the range (plus two words the reader looks ahead into) in the
 // contiguous layout; words at or past the stream's capacity read as zero.
 #pragma once
 
@@ -147,7 +147,7 @@ ZX_HD bool fx_enc_store(uint64_t *words, uint64_t nwords, const uint64_t *lds, c
 
 // ------------------------------------------------------------------ decode
 
-// phase 1: stage word i (i < nw + 1)
+// phase 1: stage word i (i < nw + 2: the reader looks up to two words ahead)
 ZX_HD void fx_dec_stage(uint64_t *lds, const uint64_t *words, uint64_t nwords, const FxPlan &p,
                         const FxGroup &q, uint32_t i)
 {

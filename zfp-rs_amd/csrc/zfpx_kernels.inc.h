@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void k_decode_fx(DecodeArgs a, FxPlan p)
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
   const FxGroup q = fx_group(blockIdx.x, p.T, a.field.nblocks, a.start, a.fixed);
-  for (uint32_t i = t; i < q.nw + 1; i += p.T) fx_dec_stage(lds, a.words, a.nwords, p, q, i);
+  for (uint32_t i = t; i < q.nw + 2; i += p.T) fx_dec_stage(lds, a.words, a.nwords, p, q, i);
   __syncthreads();
   if (t >= q.nb) return;
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
