@@ -679,27 +679,29 @@ bool contiguous(const Plan &p)
 
 namespace {
 
+typedef hipError_t (*EncFn)(const zx::EncodeArgs &, hipStream_t);
+typedef hipError_t (*DecFn)(const zx::DecodeArgs &, uint32_t *, hipStream_t);
+
+#define ZX_ROW(T) {zx_enc_##T##_1_0, zx_enc_##T##_1_1, zx_enc_##T##_2_0, zx_enc_##T##_2_1, \
+                   zx_enc_##T##_3_0, zx_enc_##T##_3_1}
+const EncFn kEnc[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
+#undef ZX_ROW
+#define ZX_ROW(T) {zx_dec_##T##_1_0, zx_dec_##T##_1_1, zx_dec_##T##_2_0, zx_dec_##T##_2_1, \
+                   zx_dec_##T##_3_0, zx_dec_##T##_3_1}
+const DecFn kDec[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
+#undef ZX_ROW
+
 hipError_t launch_encode(zfp_type t, int dims, bool rev, const zx::EncodeArgs &a, hipStream_t s)
 {
-  switch (t) {
-  case zfp_type_int32: return zx_launch_encode_i32(dims, rev, a, s);
-  case zfp_type_int64: return zx_launch_encode_i64(dims, rev, a, s);
-  case zfp_type_float: return zx_launch_encode_f32(dims, rev, a, s);
-  case zfp_type_double: return zx_launch_encode_f64(dims, rev, a, s);
-  default: return hipErrorInvalidValue;
-  }
+  if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 3) return hipErrorInvalidValue;
+  return kEnc[t - zfp_type_int32][2 * (dims - 1) + (rev ? 1 : 0)](a, s);
 }
 
 hipError_t launch_decode(zfp_type t, int dims, bool rev, const zx::DecodeArgs &a, uint32_t *walk,
                          hipStream_t s)
 {
-  switch (t) {
-  case zfp_type_int32: return zx_launch_decode_i32(dims, rev, a, walk, s);
-  case zfp_type_int64: return zx_launch_decode_i64(dims, rev, a, walk, s);
-  case zfp_type_float: return zx_launch_decode_f32(dims, rev, a, walk, s);
-  case zfp_type_double: return zx_launch_decode_f64(dims, rev, a, walk, s);
-  default: return hipErrorInvalidValue;
-  }
+  if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 3) return hipErrorInvalidValue;
+  return kDec[t - zfp_type_int32][2 * (dims - 1) + (rev ? 1 : 0)](a, walk, s);
 }
 
 uint64_t cfg_key(const zx::Cfg &c, const Plan &p)

@@ -35,16 +35,20 @@ struct DecodeArgs {
 
 }  // namespace zx
 
-// per-type launchers (zfpx_kernels_{i32,i64,f32,f64}.hip)
-#define ZX_DECLARE(NAME)                                                                     \
-  hipError_t zx_launch_encode_##NAME(int dims, bool rev, const zx::EncodeArgs &a, hipStream_t s); \
-  hipError_t zx_launch_decode_##NAME(int dims, bool rev, const zx::DecodeArgs &a, uint32_t *walk, \
-                                     hipStream_t s);
-ZX_DECLARE(i32)
-ZX_DECLARE(i64)
+// kernel launchers, one pair per (type, dimensionality, reversible):
+// zx_{enc,dec}_<f32|f64|i32|i64>_<1|2|3>_<0|1> (zfpx_kernels.hip)
+#define ZX_DECLARE1(T, D, R)                                                              \
+  hipError_t zx_enc_##T##_##D##_##R(const zx::EncodeArgs &a, hipStream_t s);              \
+  hipError_t zx_dec_##T##_##D##_##R(const zx::DecodeArgs &a, uint32_t *walk, hipStream_t s);
+#define ZX_DECLARE(T)                                                                     \
+  ZX_DECLARE1(T, 1, 0) ZX_DECLARE1(T, 1, 1) ZX_DECLARE1(T, 2, 0) ZX_DECLARE1(T, 2, 1)     \
+  ZX_DECLARE1(T, 3, 0) ZX_DECLARE1(T, 3, 1)
 ZX_DECLARE(f32)
 ZX_DECLARE(f64)
+ZX_DECLARE(i32)
+ZX_DECLARE(i64)
 #undef ZX_DECLARE
+#undef ZX_DECLARE1
 
 // zfpx_scan.hip
 // offsets[i] = start + sum(lengths[0..i)), offsets[n] = end; `tiles` holds

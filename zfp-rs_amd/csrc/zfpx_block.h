@@ -498,11 +498,15 @@ struct CountSink {
 // Word memories for sources: global words with a bound (past it reads zero,
 // bitstream/core.rs:151-157), or a staged copy addressed with a skew.
 struct GlobalWords {
+  static constexpr bool kReloadEachStep = false;
   const uint64_t *w;
   uint64_t n;
   ZX_HD uint64_t operator()(uint64_t i) const { return i < n ? w[i] : 0ull; }
 };
 struct StagedWords {
+  // LDS loads are cheap enough to refresh the second lookahead word on every
+  // step, which keeps the refill free of branches
+  static constexpr bool kReloadEachStep = true;
   const uint64_t *w;
   uint32_t skew;
   ZX_HD uint64_t operator()(uint64_t i) const
@@ -551,8 +555,9 @@ struct Source {
       buf = (nxt >> 1) >> (63 - bits);
       idx++;
       nxt = nxt2;
-      nxt2 = mem(idx + 1);
+      if constexpr (!Mem::kReloadEachStep) nxt2 = mem(idx + 1);
     }
+    if constexpr (Mem::kReloadEachStep) nxt2 = mem(idx + 1);
   }
   ZX_HD uint64_t read(uint32_t n)
   {
@@ -703,23 +708,23 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
             m = umin(n, bits);
             bits -= m;
             uint64_t x = r.read(m);
-            while (bits != 0 && n < (uint32_t)SIZE) {
-              uint64_t ahead = r.peek();
+            // group tests, one one-bit (or the final negative test) per
+            // step, written without branches so the wave's lanes stay in step
+            bool go = bits != 0 && n < (uint32_t)SIZE;
+            while (go) {
+              const uint64_t ahead = r.peek();
+              const bool one = (ahead & 1u) != 0;
               bits -= 1;
-              if ((ahead & 1u) == 0) {
-                r.consume(1);
-                m = SIZE;
-                break;
-              }
-              uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
-              uint32_t zeros = ctz64(ahead >> 1);
-              uint32_t adv = zeros < limit ? zeros : limit;
-              r.consume(adv + 1 + (zeros < limit ? 1u : 0u));
-              bits -= adv + (zeros < limit ? 1u : 0u);
-              n += adv;
-              x |= 1ull << n;
-              n += 1;
-              m = n;
+              const uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
+              const uint32_t zeros = ctz64(ahead >> 1);
+              const bool hit = zeros < limit;
+              const uint32_t adv = hit ? zeros : limit;
+              r.consume(one ? adv + 1 + (hit ? 1u : 0u) : 1u);
+              bits -= one ? adv + (hit ? 1u : 0u) : 0u;
+              x |= one ? 1ull << (n + adv) : 0ull;
+              n += one ? adv + 1 : 0u;
+              m = one ? n : (uint32_t)SIZE;
+              go = one && bits != 0 && n < (uint32_t)SIZE;
             }
             pl[i] = (PW)x;
           }

@@ -229,45 +229,4 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
   return hipGetLastError();
 }
 
-template <class S>
-static hipError_t launch_encode_type(int dims, bool rev, const EncodeArgs &a, hipStream_t s)
-{
-  switch (dims * 2 + (rev ? 1 : 0)) {
-  case 2: return launch_encode_t<S, 1, false>(a, s);
-  case 3: return launch_encode_t<S, 1, true>(a, s);
-  case 4: return launch_encode_t<S, 2, false>(a, s);
-  case 5: return launch_encode_t<S, 2, true>(a, s);
-  case 6: return launch_encode_t<S, 3, false>(a, s);
-  case 7: return launch_encode_t<S, 3, true>(a, s);
-  default: return hipErrorInvalidValue;
-  }
-}
-
-template <class S>
-static hipError_t launch_decode_type(int dims, bool rev, const DecodeArgs &a, uint32_t *walk,
-                                     hipStream_t s)
-{
-  switch (dims * 2 + (rev ? 1 : 0)) {
-  case 2: return launch_decode_t<S, 1, false>(a, walk, s);
-  case 3: return launch_decode_t<S, 1, true>(a, walk, s);
-  case 4: return launch_decode_t<S, 2, false>(a, walk, s);
-  case 5: return launch_decode_t<S, 2, true>(a, walk, s);
-  case 6: return launch_decode_t<S, 3, false>(a, walk, s);
-  case 7: return launch_decode_t<S, 3, true>(a, walk, s);
-  default: return hipErrorInvalidValue;
-  }
-}
-
 }  // namespace zx
-
-#define ZX_INSTANTIATE(S, NAME)                                                              \
-  hipError_t zx_launch_encode_##NAME(int dims, bool rev, const zx::EncodeArgs &a,           \
-                                     hipStream_t s)                                        \
-  {                                                                                        \
-    return zx::launch_encode_type<S>(dims, rev, a, s);                                      \
-  }                                                                                        \
-  hipError_t zx_launch_decode_##NAME(int dims, bool rev, const zx::DecodeArgs &a,           \
-                                     uint32_t *walk, hipStream_t s)                        \
-  {                                                                                        \
-    return zx::launch_decode_type<S>(dims, rev, a, walk, s);                                \
-  }
