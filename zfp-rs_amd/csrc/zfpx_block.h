@@ -371,6 +371,7 @@ ZX_HD uint32_t append128(uint64_t &buf, uint32_t &bits, uint64_t lo, uint64_t hi
 // word belongs to this block alone and is stored whole.  Words at or past
 // `cap` are dropped, as the reference drops writes past a stream's end.
 struct WordSink {
+  static constexpr bool kSpillPastBudget = false;
   uint64_t *words;
   uint64_t cap;
   uint64_t idx;
@@ -431,6 +432,10 @@ struct WordSink {
 // that share boundary words with their neighbours.
 template <bool SHARED>
 struct LdsSink {
+  // Without SHARED the sink owns its block's whole words and one spare word
+  // after them, and blocks end on a word boundary: bits of a code past the
+  // budget may be appended unmasked, since they only reach the spare word.
+  static constexpr bool kSpillPastBudget = !SHARED;
   uint64_t *base;
   uint32_t idx;
   uint32_t skew;
@@ -488,6 +493,7 @@ struct LdsSink {
 
 // Counts bits only (the length pass of variable-rate compression).
 struct CountSink {
+  static constexpr bool kSpillPastBudget = true;
   uint64_t n;
   ZX_HD void put(uint64_t, uint32_t k) { n += k; }
   ZX_HD void put128(uint64_t, uint64_t, uint32_t k) { n += k; }
@@ -590,34 +596,29 @@ ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, 
   clo = x & lowmask(n);
   if (rest == 0) return n + 1;  // one negative group test
   // E holds one-bit j (input position q_j) at q_j + j; the group tests then
-  // read 1, E << 1 (the one-bits) and E << 2 less its top (the positive
-  // tests between them)
-  uint64_t elo = 0, ehi = 0, r = rest;
+  // read 1, E << 1 (the one-bits) and P << 2, P being E less its top bit
+  // (the positive tests between them)
+  uint64_t elo = 0, ehi = 0, tlo = 0, thi = 0, r = rest;
   uint32_t j = 0, q = 0;
   while (r) {
     q = ctz64(r);
     const uint32_t b = q + j;
     const uint64_t t = 1ull << (b & 63);
-    elo |= b < 64 ? t : 0ull;
-    ehi |= b < 64 ? 0ull : t;
+    tlo = b < 64 ? t : 0ull;
+    thi = b < 64 ? 0ull : t;
+    elo |= tlo;
+    ehi |= thi;
     r &= r - 1;
     j++;
   }
-  const uint32_t tb = q + j - 1;
-  const uint64_t tm = 1ull << (tb & 63);
-  const uint64_t plo = tb < 64 ? elo & ~tm : elo;
-  const uint64_t phi = tb < 64 ? ehi : ehi & ~tm;
-  uint64_t glo = 1ull | (elo << 1) | (plo << 2);
-  uint64_t ghi = (ehi << 1) | (elo >> 63) | (phi << 2) | (plo >> 62);
-  uint32_t glen = tb + 3;
-  if (n + q == (uint32_t)SIZE - 1) {
-    // the one-bit at the last position is implied, with no test after
-    const uint32_t d = tb + 1;
-    const uint64_t dm = 1ull << (d & 63);
-    glo = d < 64 ? glo & ~dm : glo;
-    ghi = d < 64 ? ghi : ghi & ~dm;
-    glen = tb + 1;
-  }
+  const uint32_t tb = q + j - 1;  // position of E's top bit
+  const uint64_t plo = elo ^ tlo, phi = ehi ^ thi;
+  // a one-bit at the last position is implied, with no test after it
+  const bool implied = n + q == (uint32_t)SIZE - 1;
+  const uint64_t olo = implied ? plo : elo, ohi = implied ? phi : ehi;
+  const uint64_t glo = 1ull | (olo << 1) | (plo << 2);
+  const uint64_t ghi = (ohi << 1) | (olo >> 63) | (phi << 2) | (plo >> 62);
+  const uint32_t glen = implied ? tb + 1 : tb + 3;
   clo |= glo << n;
   chi = (ghi << n) | ((glo >> 1) >> (63 - n));
   nn = n + q + 1;
@@ -662,7 +663,10 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
         uint32_t nn;
         const uint32_t len = plane_code<SIZE>((uint64_t)ch[i], n, clo, chi, nn);
         const uint32_t take = umin(len, bits);
-        w.put128(clo & lowmask(take), take > 64 ? chi & lowmask(take - 64) : 0ull, take);
+        if constexpr (Sink::kSpillPastBudget)
+          w.put128(clo, chi, take);
+        else
+          w.put128(clo & lowmask(take), take > 64 ? chi & lowmask(take - 64) : 0ull, take);
         bits -= take;
         n = nn;
       }
