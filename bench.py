@@ -75,6 +75,25 @@ def cpu_baseline(field_gpu, rate, gpu_words_fn, budget_s=12.0):
             "stream_matches_gpu": bool(same)}
 
 
+KERNELS = {"encode": "zx::k_encode_fx<float, 3, false, true>",
+           "decode": "zx::k_decode_fx<float, 3, false>"}
+
+
+def measured_traffic(which, n, rate):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    profile (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes, each scaled by
+    what the counter reads for a 512 MiB device copy), or None when no
+    profile covers this workload."""
+    path = ROOT / "profiles" / f"traffic_{n}cube_f32_rate{rate:g}.json"
+    if not path.exists():
+        return None
+    prof = json.loads(path.read_text())
+    k = prof["kernels"].get(KERNELS[which])
+    if k is None:
+        return None
+    return k["hbm_bytes"], f"{path.relative_to(ROOT)} ({KERNELS[which]})"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,8 +106,8 @@ def main():
                     help="also run one device-to-device copy of the field (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X "
-              "(see DESIGN.md for how the codec would shard); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the codec runs on a single MI355X here and "
+              "no multi-GPU figure is produced; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -147,6 +166,7 @@ def main():
     dominant = "encode" if enc >= dec else "decode"
     dom_ms = max(enc, dec)
     achieved = alg / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dominant, n, args.rate)
     # sanity: a decode of the compressed stream round-trips the block count
     blocks = ((n + 3) // 4) ** 3
     assert stream_bytes == (blocks * cfg.maxbits + 63) // 64 * 8
@@ -175,7 +195,9 @@ def main():
         "decode_kernel_ms": round(dec, 4),
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic[0] if traffic else None,
+                     "traffic_source": traffic[1] if traffic else None,
                      "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": None,
     }

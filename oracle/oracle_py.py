@@ -89,6 +89,20 @@ def lib():
         L.orc_maximum_size.restype = ctypes.c_size_t
         L.orc_num_blocks.argtypes = [P(Field)]
         L.orc_num_blocks.restype = ctypes.c_size_t
+        vp, ci, cu = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32
+        for name, args, res in (
+                ("orc_exponent_block_f32", [vp, ci], ci), ("orc_exponent_block_f64", [vp, ci], ci),
+                ("orc_fwd_cast_f32", [vp, vp, ci, ci], None), ("orc_fwd_cast_f64", [vp, vp, ci, ci], None),
+                ("orc_inv_cast_f32", [vp, vp, ci, ci], None), ("orc_inv_cast_f64", [vp, vp, ci, ci], None),
+                ("orc_truncate_f64", [ctypes.c_double], ctypes.c_int64),
+                ("orc_precision_f", [ci, cu, ctypes.c_int32, ci, ci], cu),
+                ("orc_xform_i32", [vp, ci, ci], None), ("orc_xform_i64", [vp, ci, ci], None),
+                ("orc_fwd_round_i32", [vp, ci, cu], None), ("orc_fwd_round_i64", [vp, ci, cu], None),
+                ("orc_inv_round_u32", [vp, ci, cu, cu], None), ("orc_inv_round_u64", [vp, ci, cu, cu], None),
+                ("orc_to_planes", [vp, ci, vp], None)):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
         _lib = L
     return _lib
 

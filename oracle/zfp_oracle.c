@@ -189,10 +189,12 @@ void orc_skip(orc_stream *s, uint64_t n) { orc_rseek(s, orc_rtell(s) + n); }
 /* pad_impl (core.rs:460-478) */
 void orc_pad(orc_stream *s, uint64_t n)
 {
-  uint64_t bits = (uint64_t)s->bits + n;
+  /* saturating, as pad_impl (core.rs:460-478) */
+  uint64_t bits = n > UINT64_MAX - s->bits ? UINT64_MAX : (uint64_t)s->bits + n;
   while (bits >= 64) {
     if (s->word_pos >= s->nwords) {
-      s->word_pos += (size_t)(bits / 64);
+      const size_t words = (size_t)(bits / 64);
+      s->word_pos = words > SIZE_MAX - s->word_pos ? SIZE_MAX : s->word_pos + words;
       s->buffer = 0;
       s->overflowed = 1;
       bits %= 64;
@@ -753,3 +755,48 @@ size_t orc_decompress_threads(orc_stream *s, const orc_config *c, const orc_fiel
   if (orc_overread(s)) { *err = 3; return 0; }
   return required;
 }
+
+/* ======================= test hooks ======================= */
+/* The per-block primitives, exported for the closed-form checks of the test
+ * suite (exponent vs frexp, exact casts, transform matrices, rounding
+ * biases, plane transpose).  size is the block's value count (4, 16, 64). */
+
+int orc_exponent_block_f32(const float *f, int size) { return exponent_block_32(f, size); }
+int orc_exponent_block_f64(const double *f, int size) { return exponent_block_64(f, size); }
+void orc_fwd_cast_f32(int32_t *i, const float *f, int size, int emax) { fwd_cast_32(i, f, size, emax); }
+void orc_fwd_cast_f64(int64_t *i, const double *f, int size, int emax) { fwd_cast_64(i, f, size, emax); }
+void orc_inv_cast_f32(const int32_t *i, float *f, int size, int emax) { inv_cast_32(i, f, size, emax); }
+void orc_inv_cast_f64(const int64_t *i, double *f, int size, int emax) { inv_cast_64(i, f, size, emax); }
+int64_t orc_truncate_f64(double v) { return truncate_64(v); }
+uint32_t orc_precision_f(int emax, uint32_t maxprec, int32_t minexp, int dims, int tight)
+{
+  return precision_32(emax, maxprec, minexp, dims, tight);
+}
+
+/* which: 0 fwd_xform, 1 inv_xform, 2 rev_fwd_xform, 3 rev_inv_xform */
+void orc_xform_i32(int32_t *p, int size, int which)
+{
+  switch (which) {
+  case 0: fwd_xform_32(p, size, fwd_lift_32); break;
+  case 1: inv_xform_32(p, size, inv_lift_32); break;
+  case 2: fwd_xform_32(p, size, rev_fwd_lift_32); break;
+  default: inv_xform_32(p, size, rev_inv_lift_32); break;
+  }
+}
+void orc_xform_i64(int64_t *p, int size, int which)
+{
+  switch (which) {
+  case 0: fwd_xform_64(p, size, fwd_lift_64); break;
+  case 1: inv_xform_64(p, size, inv_lift_64); break;
+  case 2: fwd_xform_64(p, size, rev_fwd_lift_64); break;
+  default: inv_xform_64(p, size, rev_inv_lift_64); break;
+  }
+}
+
+void orc_fwd_round_i32(int32_t *b, int size, uint32_t maxprec) { fwd_round_32(b, size, maxprec); }
+void orc_fwd_round_i64(int64_t *b, int size, uint32_t maxprec) { fwd_round_64(b, size, maxprec); }
+void orc_inv_round_u32(uint32_t *d, int size, uint32_t m, uint32_t prec) { inv_round_32(d, size, m, prec); }
+void orc_inv_round_u64(uint64_t *d, int size, uint32_t m, uint32_t prec) { inv_round_64(d, size, m, prec); }
+
+/* planes[k] bit i = bit k of u[i] (u given as 64-bit values) */
+void orc_to_planes(const uint64_t *u, int size, uint64_t planes[64]) { to_planes_64(u, size, planes); }

@@ -118,6 +118,23 @@ int orc_block_offsets(orc_stream *s, const orc_config *c, const orc_field *f,
 
 size_t orc_num_blocks(const orc_field *f);
 
+/* ---- test hooks: per-block primitives for the closed-form checks ---- */
+int orc_exponent_block_f32(const float *f, int size);
+int orc_exponent_block_f64(const double *f, int size);
+void orc_fwd_cast_f32(int32_t *i, const float *f, int size, int emax);
+void orc_fwd_cast_f64(int64_t *i, const double *f, int size, int emax);
+void orc_inv_cast_f32(const int32_t *i, float *f, int size, int emax);
+void orc_inv_cast_f64(const int64_t *i, double *f, int size, int emax);
+int64_t orc_truncate_f64(double v);
+uint32_t orc_precision_f(int emax, uint32_t maxprec, int32_t minexp, int dims, int tight);
+void orc_xform_i32(int32_t *p, int size, int which);
+void orc_xform_i64(int64_t *p, int size, int which);
+void orc_fwd_round_i32(int32_t *b, int size, uint32_t maxprec);
+void orc_fwd_round_i64(int64_t *b, int size, uint32_t maxprec);
+void orc_inv_round_u32(uint32_t *d, int size, uint32_t m, uint32_t prec);
+void orc_inv_round_u64(uint64_t *d, int size, uint32_t m, uint32_t prec);
+void orc_to_planes(const uint64_t *u, int size, uint64_t planes[64]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -214,3 +214,33 @@ def test_header_round_trip_and_metadata():
     assert ZL.zfp_field_is_contiguous(ctypes.byref(f)) == 1
     ZL.zfp_stream_close(zs)
     ZL.stream_close(bs)
+
+
+def test_stream_pad_and_copy_vectors():
+    """src/bitstream.rs:964-972 (a huge pad returns at once, dropping the words
+    past the end) and :1025-1050 (copy across words), through the C ABI."""
+    ZL = Z.lib()
+    word1, word2 = (1 << 64) - 1, 0x5555555555555555
+    buf = np.zeros(3, dtype=np.uint64)
+    bs = ZL.stream_open(buf.ctypes.data, buf.nbytes)
+    ZL.stream_write_bits(bs, 3, 2)
+    ZL.stream_pad(bs, (1 << 64) - 1)
+    assert int(buf[0]) == 3 and not buf[1:].any()
+    ZL.stream_close(bs)
+    sbuf = np.zeros(3, dtype=np.uint64)
+    dbuf = np.zeros(3, dtype=np.uint64)
+    src = ZL.stream_open(sbuf.ctypes.data, sbuf.nbytes)
+    dst = ZL.stream_open(dbuf.ctypes.data, dbuf.nbytes)
+    ZL.stream_write_bits(src, word1, 64)
+    ZL.stream_write_bits(src, word2, 64)
+    ZL.stream_flush(src)
+    ZL.stream_rseek(src, 58)
+    ZL.stream_wseek(dst, 5)
+    ZL.stream_copy(dst, src, 68)
+    w2_bits = 5 + (64 - 58)
+    assert int(dbuf[0]) == (((word1 >> 58) << 5) + (word2 << w2_bits)) & word1
+    assert ZL.stream_wtell(dst) == 5 + 68
+    ZL.stream_flush(dst)
+    assert int(dbuf[1]) == (word2 >> w2_bits) & ((1 << ((5 + 68) % 64)) - 1)
+    ZL.stream_close(src)
+    ZL.stream_close(dst)

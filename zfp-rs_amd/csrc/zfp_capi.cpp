@@ -223,12 +223,15 @@ struct bitstream {
   uint64_t rtell() const { return (uint64_t)word_pos * 64 - bits; }
   void skip(uint64_t n) { rseek(rtell() + n); }
   // pad_impl (core.rs:460-478)
+  // pad_impl (core.rs:460-478): the count saturates, and words past the end
+  // are skipped at once
   void pad(uint64_t n)
   {
-    uint64_t b = (uint64_t)bits + n;
+    uint64_t b = n > UINT64_MAX - bits ? UINT64_MAX : (uint64_t)bits + n;
     while (b >= 64) {
       if (word_pos >= nwords) {
-        word_pos += (size_t)(b / 64);
+        const size_t words = (size_t)(b / 64);
+        word_pos = words > SIZE_MAX - word_pos ? SIZE_MAX : word_pos + words;
         buffer = 0;
         overflowed = true;
         b %= 64;
