@@ -51,6 +51,21 @@ size_t zfp_stream_index_export(const bitstream *stream, uint32_t *lengths, size_
 zfp_bool zfp_stream_index_import(bitstream *stream, const uint32_t *lengths, size_t count,
                                  uint64 start_bit);
 
+/* Batched codec, for callers that code many independent arrays at once
+ * (serving, checkpoint writers).  Equivalent to zfp_compress(streams[i],
+ * fields[i]) (or zfp_decompress) for i < count, in order: the same stream
+ * bytes, cursors and return values (in sizes[i]; 0 on failure).  Arrays whose
+ * data and stream words are in device memory are coded together with a few
+ * kernel launches per phase; the others, and a stream's second and later
+ * arrays, are coded by the single-array call after the batch.
+ * ZFP_HIP_BATCH_DEVICE_FIELDS asserts that every field's data is device
+ * memory (skips the pointer queries).  Returns the number of arrays coded. */
+#define ZFP_HIP_BATCH_DEVICE_FIELDS 1u
+size_t zfp_hip_compress_batch(zfp_stream *const *streams, const zfp_field *const *fields, size_t count,
+                              size_t *sizes, uint flags);
+size_t zfp_hip_decompress_batch(zfp_stream *const *streams, zfp_field *const *fields, size_t count,
+                                size_t *sizes, uint flags);
+
 /* Device the library runs on (the current HIP device when first used). */
 int zfp_hip_device(void);
 

@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <mutex>
+#include <unordered_set>
+#include <vector>
 
 #include "../../include/zfp.h"
 #include "../../include/zfp_hip.h"
@@ -966,6 +968,284 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
 
 }  // namespace
 
+// ===================================================================== batches
+
+namespace {
+
+typedef hipError_t (*BEncFn)(const zx::BatchArgs &, bool, hipStream_t);
+typedef hipError_t (*BDecFn)(const zx::BatchArgs &, bool, hipStream_t);
+
+#define ZX_ROW(T) {zx_benc_##T##_1_0, zx_benc_##T##_1_1, zx_benc_##T##_2_0, zx_benc_##T##_2_1, \
+                   zx_benc_##T##_3_0, zx_benc_##T##_3_1}
+const BEncFn kBEnc[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
+#undef ZX_ROW
+#define ZX_ROW(T) {zx_bdec_##T##_1_0, zx_bdec_##T##_1_1, zx_bdec_##T##_2_0, zx_bdec_##T##_2_1, \
+                   zx_bdec_##T##_3_0, zx_bdec_##T##_3_1}
+const BDecFn kBDec[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
+#undef ZX_ROW
+
+// One array of a batch, on the host side.
+struct BatchItem {
+  size_t input;      // position in the caller's arrays
+  int cls;           // ((type, dims, rev) variant) * 2 + variable placement
+  Plan plan;
+  zx::BatchJob job;
+  uint64_t w0;       // first stream word of the array
+  uint64_t start;    // cursor bit at the call
+  size_t seek_word0; // decode: seek word at the call
+};
+
+// Grow-only pinned and device buffers of the batch entry points.
+struct BatchBuffers {
+  void *hbuf = nullptr; size_t hcap = 0;  // pinned: job table + firsts, then results
+  void *dbuf = nullptr; size_t dcap = 0;  // device copy of the same
+};
+
+BatchBuffers &batch_buffers()
+{
+  static BatchBuffers b;
+  return b;
+}
+
+bool grow_pinned(void *&p, size_t &cap, size_t need)
+{
+  if (need <= cap) return true;
+  const size_t n = std::max(need, cap + cap / 2);
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    return false;
+  }
+  cap = n;
+  return true;
+}
+
+// Checks an array as zfp_compress / zfp_decompress do; false when the call
+// would return 0 before coding.
+bool batch_plan(zfp_stream *z, const zfp_field *f, bool decode, Plan &p)
+{
+  if (!z || !f || !z->stream || !f->data) return false;
+  if (decode && !valid_type(f->type)) return false;
+  if (!make_plan(f, p) || p.dims > 3) return false;
+  if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return false;
+  return true;
+}
+
+size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_t count, size_t *sizes,
+                 uint flags)
+{
+  if (!zs || !fs || !sizes) return 0;
+  size_t ok = 0;
+  std::vector<BatchItem> items;
+  std::vector<size_t> later;  // arrays for the single-call path, in input order
+  std::unordered_set<const bitstream *> seen;
+  items.reserve(count);
+  for (size_t i = 0; i < count; i++) {
+    sizes[i] = 0;
+    zfp_stream *z = zs[i];
+    zfp_field *f = fs[i];
+    Plan p;
+    if (!batch_plan(z, f, !encode, p)) continue;
+    bitstream *s = z->stream;
+    if (!seen.insert(s).second) {
+      later.push_back(i);
+      continue;
+    }
+    if (p.nblocks == 0) {  // nothing to code: the calls' own handling
+      later.push_back(i);
+      continue;
+    }
+    const bool dev = s->device && ((flags & ZFP_HIP_BATCH_DEVICE_FIELDS) || is_device_ptr(f->data));
+    if (!dev) {
+      later.push_back(i);
+      continue;
+    }
+    BatchItem it;
+    it.input = i;
+    it.plan = p;
+    const zx::Cfg c = cfg_of(z);
+    const bool rev = c.minexp < kMinExp;
+    const bool fixed = fixed_layout(c, p);
+    if (encode) {
+      it.start = s->wtell();
+    } else {
+      s->reset_overread();
+      it.seek_word0 = s->seek_word;
+      it.start = s->rtell();
+    }
+    it.w0 = it.start / 64;
+    zx::BatchJob &J = it.job;
+    memset(&J, 0, sizeof(J));
+    J.field = field_desc(p, f->data);
+    J.cfg = c;
+    J.vec = vec_ok(p, f->data);
+    J.words = s->words + it.w0;
+    J.nwords = s->nwords > it.w0 ? s->nwords - it.w0 : 0;
+    J.start = it.start - 64 * it.w0;
+    J.prefix = encode && s->bits ? s->buffer : 0;
+    J.fixed = fixed ? c.maxbits : 0;
+    J.fill = !(fixed && J.start == 0 && (c.maxbits % 64) == 0);
+    if (!fixed) {
+      if (encode) {
+        // keep the offset index beside the stream, as zfp_compress does
+        if (s->index_cap < p.nblocks) {
+          drop_index(s);
+          if (hipMalloc((void **)&s->index, p.nblocks * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            s->index = nullptr;
+          } else {
+            s->index_cap = p.nblocks;
+          }
+        }
+        J.index_out = s->index;
+      } else if (s->index && s->index_n == p.nblocks && s->index_start == it.start &&
+                 (s->index_key == 0 || s->index_key == cfg_key(c, p))) {
+        J.index = s->index;
+      }
+    }
+    it.cls = (((int)p.type - (int)zfp_type_int32) * 6 + (p.dims - 1) * 2 + (rev ? 1 : 0)) * 2 +
+             (fixed ? 0 : 1);
+    items.push_back(it);
+  }
+
+  if (!items.empty()) {
+    std::stable_sort(items.begin(), items.end(),
+                     [](const BatchItem &a, const BatchItem &b) { return a.cls < b.cls; });
+    const size_t n = items.size();
+    std::vector<uint64_t> firsts(n);
+    uint64_t var_total = 0;
+    for (size_t k = 0; k < n; k++) {
+      if (!items[k].job.fixed) {
+        items[k].job.lbase = var_total;
+        var_total += items[k].plan.nblocks;
+      }
+    }
+    for (size_t k = 0; k < n;) {
+      size_t e = k;
+      uint64_t acc = 0;
+      while (e < n && items[e].cls == items[k].cls) {
+        firsts[e] = acc;
+        acc += items[e].plan.nblocks;
+        e++;
+      }
+      k = e;
+    }
+    zfp_stream *z0 = zs[items[0].input];
+    const hipStream_t st = hip_stream_of(z0);
+    Workspace &W = ws();
+    BatchBuffers &B = batch_buffers();
+    std::lock_guard<std::mutex> lock(W.mu);
+    const size_t jbytes = (n * sizeof(zx::BatchJob) + 15) / 16 * 16;
+    const size_t fbytes = n * 8;
+    const size_t rbytes = n * 16;
+    bool good = grow_pinned(B.hbuf, B.hcap, std::max(jbytes + fbytes, rbytes)) &&
+                Workspace::grow(B.dbuf, B.dcap, jbytes + fbytes + rbytes);
+    if (good && var_total) {
+      good = Workspace::grow(W.lens, W.lens_cap, var_total * 4) &&
+             Workspace::grow(W.offs, W.offs_cap, (var_total + 1) * 8) &&
+             Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(var_total) * 8);
+    }
+    if (good) {
+      uint64_t *dres = (uint64_t *)((char *)B.dbuf + jbytes + fbytes);
+      zx::BatchJob *hjobs = (zx::BatchJob *)B.hbuf;
+      for (size_t k = 0; k < n; k++) {
+        items[k].job.result = dres + 2 * k;
+        hjobs[k] = items[k].job;
+      }
+      memcpy((char *)B.hbuf + jbytes, firsts.data(), fbytes);
+      const zx::BatchJob *djobs = (const zx::BatchJob *)B.dbuf;
+      const uint64_t *dfirsts = (const uint64_t *)((char *)B.dbuf + jbytes);
+      uint32_t *lens = (uint32_t *)W.lens;
+      uint64_t *offs = (uint64_t *)W.offs;
+      auto run = [&](bool var_only, bool count) -> bool {
+        for (size_t k = 0; k < n;) {
+          size_t e = k;
+          uint64_t nb = 0;
+          while (e < n && items[e].cls == items[k].cls) nb += items[e++].plan.nblocks;
+          const int cls = items[k].cls;
+          if (!var_only || (cls & 1)) {
+            zx::BatchArgs a;
+            a.jobs = djobs + k;
+            a.firsts = dfirsts + k;
+            a.njobs = (uint32_t)(e - k);
+            a.nblocks = nb;
+            a.lengths = lens;
+            a.offsets = offs;
+            const int v = cls >> 1;
+            const hipError_t r = encode ? kBEnc[v / 6][v % 6](a, count, st) : kBDec[v / 6][v % 6](a, count, st);
+            if (r != hipSuccess) {
+              (void)hipGetLastError();
+              return false;
+            }
+          }
+          k = e;
+        }
+        return true;
+      };
+      good = hipMemcpyAsync(B.dbuf, B.hbuf, jbytes + fbytes, hipMemcpyHostToDevice, st) == hipSuccess;
+      if (good && var_total) {
+        if (encode) good = run(true, true);
+        else good = zx_batch_gather_index(djobs, (uint32_t)n, lens, st) == hipSuccess && run(true, true);
+        good = good && zx_launch_scan(lens, var_total, 0, offs, (uint64_t *)W.tiles, st) == hipSuccess;
+      }
+      good = good && zx_batch_ends(djobs, (uint32_t)n, offs, encode, st) == hipSuccess;
+      good = good && run(false, false);
+      if (good && encode && var_total) good = zx_batch_keep_index(djobs, (uint32_t)n, lens, st) == hipSuccess;
+      good = good && hipMemcpyAsync(B.hbuf, dres, rbytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+      good = good && hipStreamSynchronize(st) == hipSuccess;
+      if (!good) (void)hipGetLastError();
+    }
+    if (good) {
+      const uint64_t *res = (const uint64_t *)B.hbuf;
+      for (size_t k = 0; k < n; k++) {
+        const BatchItem &it = items[k];
+        zfp_stream *z = zs[it.input];
+        bitstream *s = z->stream;
+        const uint64_t end = 64 * it.w0 + res[2 * k];
+        const uint64_t w1 = (end + 63) / 64;
+        if (encode) {
+          // the cursor after the last block, then flush (core.rs:536-542)
+          if (w1 > s->nwords) s->overflowed = true;
+          s->word_pos = (size_t)w1;
+          s->bits = 0;
+          s->buffer = 0;
+          if (!it.job.fixed && s->index) {
+            s->index_n = it.plan.nblocks;
+            s->index_start = it.start;
+            s->index_key = cfg_key(cfg_of(z), it.plan);
+          }
+          if (!s->overflowed) {
+            sizes[it.input] = s->word_pos * 8;
+            ok++;
+          }
+        } else {
+          const uint64_t loaded_end = it.w0 + res[2 * k + 1];
+          s->word_pos = (size_t)w1;
+          s->bits = 0;
+          s->buffer = 0;
+          s->seek_word = s->word_pos;
+          s->overread = loaded_end > std::max<uint64_t>(s->nwords, it.seek_word0);
+          if (!s->overread_now()) {
+            sizes[it.input] = s->word_pos * 8;
+            ok++;
+          }
+        }
+      }
+    }
+  }
+  std::sort(later.begin(), later.end());
+  for (size_t i : later) {
+    sizes[i] = encode ? zfp_compress(zs[i], fs[i]) : zfp_decompress(zs[i], fs[i]);
+    if (sizes[i]) ok++;
+  }
+  return ok;
+}
+
+}  // namespace
+
 // ===================================================================== C ABI
 
 extern "C" {
@@ -1586,6 +1866,25 @@ double zfp_stream_kernel_ms(const zfp_stream *z, double *call_ms)
   const ExecParams *p = z ? params(z) : nullptr;
   if (call_ms) *call_ms = p && p->timing ? p->call_ms : -1.0;
   return p && p->timing ? p->kernel_ms : -1.0;
+}
+
+// Batched codec: count independent arrays coded with a few launches per phase
+// (one per class of arrays sharing scalar type, dimensionality, reversible
+// mode and fixed or variable placement) instead of several per array.  Each
+// array's stream, cursor and return value are those of zfp_compress /
+// zfp_decompress on it; arrays the batch cannot take (host-resident data or
+// words, a stream used twice) are coded by those calls after the batch, in
+// input order.  Returns the number of arrays coded without error.
+size_t zfp_hip_compress_batch(zfp_stream *const *streams, const zfp_field *const *fields, size_t count,
+                              size_t *sizes, uint flags)
+{
+  return batch_run(true, streams, (zfp_field *const *)fields, count, sizes, flags);
+}
+
+size_t zfp_hip_decompress_batch(zfp_stream *const *streams, zfp_field *const *fields, size_t count,
+                                size_t *sizes, uint flags)
+{
+  return batch_run(false, streams, fields, count, sizes, flags);
 }
 
 int zfp_hip_device(void)

@@ -33,13 +33,44 @@ struct DecodeArgs {
   bool vec;
 };
 
+// One array of a batch (zfp_hip_compress_batch / zfp_hip_decompress_batch).
+struct BatchJob {
+  FieldDesc field;
+  Cfg cfg;
+  uint64_t *words;          // stream words from the array's first word
+  uint64_t nwords;          // capacity from there: writes past it drop, reads give zero
+  uint64_t start;           // bit of block 0 within words[0]
+  uint64_t prefix;          // encode: bits of words[0] below `start`, kept by the fill
+  uint64_t lbase;           // variable placement: block 0's slot in the batch's lengths
+  const uint32_t *index;    // decode: the stream's offset index, or null (walk the stream)
+  uint32_t *index_out;      // encode: where to keep the offset index, or null
+  uint64_t *result;         // [0] end bit of the last block; [1] decode: one past the
+                            // highest word loaded
+  uint32_t fixed;           // bits per block (fixed placement), or 0
+  uint32_t fill;            // encode: zero the words first (blocks share words)
+  bool vec;                 // rows of whole blocks may be moved as 16-byte vectors
+};
+
+// One launch over the arrays of one class (same scalar type, dimensionality,
+// reversible or not, fixed or variable placement).
+struct BatchArgs {
+  const BatchJob *jobs;
+  const uint64_t *firsts;   // firsts[j]: index of jobs[j]'s block 0 among the launch's blocks
+  uint32_t njobs;
+  uint64_t nblocks;
+  uint32_t *lengths;        // batch-wide per-block bit lengths (variable placement)
+  const uint64_t *offsets;  // batch-wide exclusive scan of `lengths`
+};
+
 }  // namespace zx
 
 // kernel launchers, one pair per (type, dimensionality, reversible):
 // zx_{enc,dec}_<f32|f64|i32|i64>_<1|2|3>_<0|1> (zfpx_kernels.hip)
 #define ZX_DECLARE1(T, D, R)                                                              \
   hipError_t zx_enc_##T##_##D##_##R(const zx::EncodeArgs &a, hipStream_t s);              \
-  hipError_t zx_dec_##T##_##D##_##R(const zx::DecodeArgs &a, uint32_t *walk, hipStream_t s);
+  hipError_t zx_dec_##T##_##D##_##R(const zx::DecodeArgs &a, uint32_t *walk, hipStream_t s); \
+  hipError_t zx_benc_##T##_##D##_##R(const zx::BatchArgs &a, bool count, hipStream_t s);  \
+  hipError_t zx_bdec_##T##_##D##_##R(const zx::BatchArgs &a, bool walk, hipStream_t s);
 #define ZX_DECLARE(T)                                                                     \
   ZX_DECLARE1(T, 1, 0) ZX_DECLARE1(T, 1, 1) ZX_DECLARE1(T, 2, 0) ZX_DECLARE1(T, 2, 1)     \
   ZX_DECLARE1(T, 3, 0) ZX_DECLARE1(T, 3, 1)
@@ -58,3 +89,13 @@ hipError_t zx_launch_scan(const uint32_t *lengths, uint64_t n, uint64_t start, u
                           uint64_t *tiles, hipStream_t s);
 // words[0] = first, words[1..n) = 0
 hipError_t zx_launch_fill(uint64_t *words, uint64_t n, uint64_t first, hipStream_t s);
+
+// Batch helpers over all `n` arrays of a batch (one workgroup per array):
+// copy each variable-rate array's offset index into the batch-wide lengths;
+hipError_t zx_batch_gather_index(const zx::BatchJob *jobs, uint32_t n, uint32_t *lengths, hipStream_t s);
+// write each array's end bit to result[0] and, for encode, zero its words
+// (keeping the prefix bits of the first) where `fill` is set;
+hipError_t zx_batch_ends(const zx::BatchJob *jobs, uint32_t n, const uint64_t *offsets, bool encode,
+                         hipStream_t s);
+// copy each variable-rate array's lengths to its index_out.
+hipError_t zx_batch_keep_index(const zx::BatchJob *jobs, uint32_t n, const uint32_t *lengths, hipStream_t s);

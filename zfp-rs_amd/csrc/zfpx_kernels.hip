@@ -17,3 +17,13 @@ hipError_t ZX_FN(dec)(const zx::DecodeArgs &a, uint32_t *walk, hipStream_t s)
 {
   return zx::launch_decode_t<ZX_S, ZX_D, ZX_REV != 0>(a, walk, s);
 }
+
+hipError_t ZX_FN(benc)(const zx::BatchArgs &a, bool count, hipStream_t s)
+{
+  return zx::launch_batch_encode_t<ZX_S, ZX_D, ZX_REV != 0>(a, count, s);
+}
+
+hipError_t ZX_FN(bdec)(const zx::BatchArgs &a, bool walk, hipStream_t s)
+{
+  return zx::launch_batch_decode_t<ZX_S, ZX_D, ZX_REV != 0>(a, walk, s);
+}

@@ -190,6 +190,87 @@ __global__ __launch_bounds__(kThreads) void k_decode_fx(DecodeArgs a, FxPlan p)
   store_block<S, D>(a.field, q.b0 + t, a.vec, v);
 }
 
+// ------------------------------------------------------------------ batches
+
+// The array holding a launch's block g: the last j with firsts[j] <= g.
+__device__ __forceinline__ uint32_t job_of(const uint64_t *firsts, uint32_t njobs, uint64_t g)
+{
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (firsts[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint64_t batch_pos(const BatchArgs &a, const BatchJob &J, uint64_t b)
+{
+  return J.fixed ? J.start + b * (uint64_t)J.fixed : J.start + (a.offsets[J.lbase + b] - a.offsets[J.lbase]);
+}
+
+// Every block of every array of one class; COUNT writes bit lengths.
+template <class S, int D, bool REV, bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_batch_encode(BatchArgs a)
+{
+  const uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (g >= a.nblocks) return;
+  const uint32_t j = job_of(a.firsts, a.njobs, g);
+  const BatchJob &J = a.jobs[j];
+  const uint64_t b = g - a.firsts[j];
+  typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+  load_block<S, D>(J.field, b, J.vec, v);
+  if constexpr (COUNT) {
+    CountSink cs{0};
+    encode_values<S, D, REV>(cs, v, J.cfg);
+    a.lengths[J.lbase + b] = (uint32_t)cs.n;
+  } else {
+    WordSink w;
+    w.init(J.words, J.nwords, batch_pos(a, J, b));
+    encode_values<S, D, REV>(w, v, J.cfg);
+    w.finish();
+  }
+}
+
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(kThreads) void k_batch_decode(BatchArgs a)
+{
+  const uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (g >= a.nblocks) return;
+  const uint32_t j = job_of(a.firsts, a.njobs, g);
+  const BatchJob &J = a.jobs[j];
+  const uint64_t b = g - a.firsts[j];
+  const uint64_t pos = batch_pos(a, J, b);
+  WordSource r;
+  r.init(GlobalWords{J.words, J.nwords}, pos);
+  typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+  decode_values<S, D, REV>(r, v, J.cfg);
+  if (b == J.field.nblocks - 1) J.result[1] = loaded_words(r, pos, J.cfg.minbits);
+  store_block<S, D>(J.field, b, J.vec, v);
+}
+
+// Index-less variable-rate arrays: one thread per array decodes its blocks in
+// order to find their lengths (as k_walk does for one stream).
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(64) void k_batch_walk(BatchArgs a)
+{
+  const uint32_t j = blockIdx.x * 64 + threadIdx.x;
+  if (j >= a.njobs) return;
+  const BatchJob &J = a.jobs[j];
+  if (J.fixed || J.index) return;
+  uint64_t pos = J.start;
+  for (uint64_t b = 0; b < J.field.nblocks; b++) {
+    WordSource r;
+    r.init(GlobalWords{J.words, J.nwords}, pos);
+    typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+    decode_values<S, D, REV>(r, v, J.cfg);
+    uint64_t used = r.pos() - pos;
+    if (used < J.cfg.minbits) used = J.cfg.minbits;
+    a.lengths[J.lbase + b] = (uint32_t)used;
+    pos += used;
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 
 template <class S, int D, bool REV>
@@ -226,6 +307,30 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
   }
   const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
   hipLaunchKernelGGL((k_decode<S, D, REV>), dim3(grid), dim3(kThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class S, int D, bool REV>
+static hipError_t launch_batch_encode_t(const BatchArgs &a, bool count, hipStream_t s)
+{
+  if (a.nblocks == 0) return hipSuccess;
+  const uint64_t grid = (a.nblocks + kThreads - 1) / kThreads;
+  if (count) hipLaunchKernelGGL((k_batch_encode<S, D, REV, true>), dim3(grid), dim3(kThreads), 0, s, a);
+  else hipLaunchKernelGGL((k_batch_encode<S, D, REV, false>), dim3(grid), dim3(kThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class S, int D, bool REV>
+static hipError_t launch_batch_decode_t(const BatchArgs &a, bool walk, hipStream_t s)
+{
+  if (walk) {
+    if (a.njobs == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_batch_walk<S, D, REV>), dim3((a.njobs + 63) / 64), dim3(64), 0, s, a);
+    return hipGetLastError();
+  }
+  if (a.nblocks == 0) return hipSuccess;
+  const uint64_t grid = (a.nblocks + kThreads - 1) / kThreads;
+  hipLaunchKernelGGL((k_batch_decode<S, D, REV>), dim3(grid), dim3(kThreads), 0, s, a);
   return hipGetLastError();
 }
 

@@ -79,6 +79,37 @@ __global__ void k_fill(uint64_t *w, uint64_t n, uint64_t first)
   if (i < n) w[i] = i == 0 ? first : 0ull;
 }
 
+// ------------------------------------------------------------------ batches
+// One workgroup per array of a batch.
+
+__global__ __launch_bounds__(kT) void k_batch_gather_index(const zx::BatchJob *jobs, uint32_t *lengths)
+{
+  const zx::BatchJob &J = jobs[blockIdx.x];
+  if (J.fixed || !J.index) return;
+  for (uint64_t b = threadIdx.x; b < J.field.nblocks; b += kT) lengths[J.lbase + b] = J.index[b];
+}
+
+__global__ __launch_bounds__(kT) void k_batch_ends(const zx::BatchJob *jobs, const uint64_t *offsets,
+                                                   bool encode)
+{
+  const zx::BatchJob &J = jobs[blockIdx.x];
+  const uint64_t n = J.field.nblocks;
+  const uint64_t end = J.fixed ? J.start + n * (uint64_t)J.fixed
+                               : J.start + (offsets[J.lbase + n] - offsets[J.lbase]);
+  if (threadIdx.x == 0) J.result[0] = end;
+  if (!encode || !J.fill) return;
+  uint64_t nw = (end + 63) / 64;
+  if (nw > J.nwords) nw = J.nwords;
+  for (uint64_t i = threadIdx.x; i < nw; i += kT) J.words[i] = i == 0 ? J.prefix : 0ull;
+}
+
+__global__ __launch_bounds__(kT) void k_batch_keep_index(const zx::BatchJob *jobs, const uint32_t *lengths)
+{
+  const zx::BatchJob &J = jobs[blockIdx.x];
+  if (J.fixed || !J.index_out) return;
+  for (uint64_t b = threadIdx.x; b < J.field.nblocks; b += kT) J.index_out[b] = lengths[J.lbase + b];
+}
+
 }  // namespace
 
 uint64_t zx_scan_tiles(uint64_t n) { return (n + kTile - 1) / kTile + 1; }
@@ -101,5 +132,27 @@ hipError_t zx_launch_fill(uint64_t *words, uint64_t n, uint64_t first, hipStream
 {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, words, n, first);
+  return hipGetLastError();
+}
+
+hipError_t zx_batch_gather_index(const zx::BatchJob *jobs, uint32_t n, uint32_t *lengths, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_gather_index, dim3(n), dim3(kT), 0, s, jobs, lengths);
+  return hipGetLastError();
+}
+
+hipError_t zx_batch_ends(const zx::BatchJob *jobs, uint32_t n, const uint64_t *offsets, bool encode,
+                         hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_ends, dim3(n), dim3(kT), 0, s, jobs, offsets, encode);
+  return hipGetLastError();
+}
+
+hipError_t zx_batch_keep_index(const zx::BatchJob *jobs, uint32_t n, const uint32_t *lengths, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_keep_index, dim3(n), dim3(kT), 0, s, jobs, lengths);
   return hipGetLastError();
 }

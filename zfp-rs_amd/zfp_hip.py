@@ -125,7 +125,10 @@ SIGNATURES = {
     "zfp_stream_set_timing": (_B, [_SP, _B]),
     "zfp_stream_kernel_ms": (_D, [_SP, P(_D)]),
     "zfp_hip_device": (ctypes.c_int, []),
+    "zfp_hip_compress_batch": (_SZ, [P(_SP), P(_FP), _SZ, P(_SZ), _U]),
+    "zfp_hip_decompress_batch": (_SZ, [P(_SP), P(_FP), _SZ, P(_SZ), _U]),
 }
+BATCH_DEVICE_FIELDS = 1
 for _name in ("zfp_config_none", "zfp_config_rate", "zfp_config_precision", "zfp_config_accuracy",
               "zfp_config_reversible", "zfp_config_expert"):
     SIGNATURES[_name] = (None, None)  # struct returns: exported, not called from Python
@@ -358,6 +361,30 @@ class Stream:
             self.close()
         except Exception:
             pass
+
+
+def _batch(fn, streams, arrays, flags):
+    n = len(streams)
+    if len(arrays) != n:
+        raise ValueError("one array per stream")
+    fields = [make_field(a) for a in arrays]
+    zs = (_SP * n)(*[s.zs for s in streams])
+    fps = (_FP * n)(*[ctypes.pointer(f) for f in fields])
+    sizes = (ctypes.c_size_t * n)()
+    done = fn(zs, fps, n, sizes, flags)
+    return done, list(sizes)
+
+
+def compress_batch(streams, arrays, flags=0):
+    """zfp_hip_compress_batch: arrays[i] into streams[i] (Stream objects), with
+    one group of launches per class of arrays.  Returns (arrays coded, the
+    per-array byte counts zfp_compress would return, 0 for a failure)."""
+    return _batch(lib().zfp_hip_compress_batch, streams, arrays, flags)
+
+
+def decompress_batch(streams, arrays, flags=0):
+    """zfp_hip_decompress_batch: the inverse of compress_batch."""
+    return _batch(lib().zfp_hip_decompress_batch, streams, arrays, flags)
 
 
 def maximum_size(config, arr):
