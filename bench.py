@@ -13,7 +13,10 @@ Also reported:
   cpu_baseline  the oracle's threaded CPU codec (C-OMP chunked compress,
                 per-chunk-seek fixed-rate decompress) on a 512x512x64 slab of
                 the same field, timed on this host, and its stream checked
-                byte for byte against the GPU's.
+                byte for byte against the GPU's;
+  batched       4096 mixed arrays (shapes, types, modes) compressed and
+                decompressed by the batched entry points: GB/s, the step's
+                p50/p99 latency, and the same work as one call per array.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rate 8] [--n 512]
 """
@@ -75,6 +78,134 @@ def cpu_baseline(field_gpu, rate, gpu_words_fn, budget_s=12.0):
             "stream_matches_gpu": bool(same)}
 
 
+def batched_leg(n_arrays, reps, seed=4096):
+    """4096 independent arrays (dims 1-3 uniform, each axis log-uniform in
+    [3, 300], type uniform over f32/f64/i32/i64, mode uniform over fixed rate
+    4-24, fixed precision 8-48 and fixed accuracy 1e-9..1e-2 log-uniform,
+    integers coded at full precision under fixed accuracy, each filled from
+    its own seed with the k^-3 field), device-resident.  One step = compress
+    then decompress every array.  Times the batched entry points against one
+    zfp_compress / zfp_decompress call per array, checks that both give the
+    same bytes, and reports the batched step's p50/p99 over `reps` steps."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import zfp_hip as Z
+    from synth import grf_tiles, grf_window
+
+    def note(msg):
+        print(f"[batched] {msg}", file=sys.stderr, flush=True)
+
+    rng = np.random.default_rng(seed)
+    tiles = grf_tiles(seed, "cuda")
+    note("field tiles ready")
+    types = [(Z.TYPE_INT32, np.int32, 2.0 ** 16), (Z.TYPE_INT64, np.int64, 2.0 ** 32),
+             (Z.TYPE_FLOAT, np.float32, 1.0), (Z.TYPE_DOUBLE, np.float64, 1.0)]
+    arrays, outs, streams, bufs = [], [], [], []
+    field_bytes = 0
+    classes = set()
+    for i in range(n_arrays):
+        dims = int(rng.integers(1, 4))
+        shape = tuple(int(round(math.exp(rng.uniform(math.log(3), math.log(300))))) for _ in range(dims))
+        zt, npdt, scale = types[int(rng.integers(4))]
+        mode = int(rng.integers(3))
+        if mode == 0:
+            cfg = Z.ZfpConfig.fixed_rate(float(rng.integers(4, 25)), zt, dims)
+        elif mode == 1:
+            cfg = Z.ZfpConfig.fixed_precision(int(rng.integers(8, 49)))
+        else:
+            cfg = Z.ZfpConfig.fixed_accuracy(10.0 ** rng.uniform(-9, -2))
+        x = grf_window(tiles, shape, seed + i, dtype=npdt, scale=scale)
+        cap = Z.maximum_size(cfg, x)
+        buf = torch.zeros((cap + 7) // 8, dtype=torch.int64, device="cuda")
+        arrays.append(x)
+        outs.append(torch.empty_like(x))
+        bufs.append(buf)
+        streams.append(Z.Stream(buf, cfg))
+        field_bytes += x.numel() * x.element_size()
+        classes.add((zt, dims, mode == 0))
+        if (i + 1) % 1024 == 0:
+            note(f"{i + 1} arrays generated")
+    del tiles
+    torch.cuda.synchronize()
+    note(f"{n_arrays} arrays, {field_bytes / 1e9:.2f} GB")
+
+    def rewind():
+        for st in streams:
+            st.rewind()
+
+    # one call per array
+    single_ms = []
+    for _ in range(2):
+        rewind()
+        t0 = time.perf_counter()
+        sizes = [st.compress(x) for st, x in zip(streams, arrays)]
+        rewind()
+        for st, o in zip(streams, outs):
+            st.decompress(o)
+        torch.cuda.synchronize()
+        single_ms.append((time.perf_counter() - t0) * 1e3)
+        note(f"one call per array: {single_ms[-1]:.1f} ms")
+    sample = list(range(0, n_arrays, max(1, n_arrays // 64)))
+    ref_words = {k: bufs[k][: sizes[k] // 8].cpu().clone() for k in sample}
+    ref_out = {k: outs[k].cpu().clone() for k in sample}
+    stream_bytes = sum(sizes)
+
+    batch = Z.Batch(streams, arrays, outs)
+    step_ms, comp_ms, decomp_ms = [], [], []
+    ok = True
+    for r in range(reps + 2):
+        rewind()
+        for k in sample:
+            bufs[k].zero_()
+            outs[k].zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nc = batch.compress()
+        t1 = time.perf_counter()
+        ok = ok and nc == n_arrays and list(batch.sizes) == sizes
+        rewind()
+        t2 = time.perf_counter()
+        nd = batch.decompress()
+        t3 = time.perf_counter()
+        ok = ok and nd == n_arrays
+        note(f"batched step {r}: {(t1 - t0 + t3 - t2) * 1e3:.2f} ms")
+        if r >= 2:
+            comp_ms.append((t1 - t0) * 1e3)
+            decomp_ms.append((t3 - t2) * 1e3)
+            step_ms.append((t1 - t0 + t3 - t2) * 1e3)
+    # the batched bytes and values equal the single calls'
+    rewind()
+    batch.compress()
+    same = ok and all(bufs[k][: sizes[k] // 8].cpu().equal(ref_words[k]) for k in sample)
+    rewind()
+    batch.decompress()
+    torch.cuda.synchronize()
+    same = same and all(outs[k].cpu().equal(ref_out[k]) for k in sample)
+    for st in streams:
+        st.close()
+    best_single = min(single_ms)
+    p50 = float(np.percentile(step_ms, 50))
+    p99 = float(np.percentile(step_ms, 99))
+    res = {
+        "arrays": n_arrays, "classes": len(classes), "uncompressed_bytes": field_bytes,
+        "compressed_bytes": stream_bytes,
+        "batched_GBps": round(2 * field_bytes / (p50 * 1e-3) / 1e9, 2),
+        "per_call_GBps": round(2 * field_bytes / (best_single * 1e-3) / 1e9, 2),
+        "speedup_vs_per_call": round(best_single / p50, 2),
+        "step_p50_ms": round(p50, 3), "step_p99_ms": round(p99, 3),
+        "compress_p50_ms": round(float(np.percentile(comp_ms, 50)), 3),
+        "decompress_p50_ms": round(float(np.percentile(decomp_ms, 50)), 3),
+        "per_call_step_ms": round(best_single, 3), "steps": reps,
+        "bytes_match_per_call": bool(same),
+    }
+    del arrays, outs, bufs, batch
+    torch.cuda.empty_cache()
+    return res
+
+
 KERNELS = {"encode": "zx::k_encode_fx<float, 3, false, true>",
            "decode": "zx::k_decode_fx<float, 3, false>"}
 
@@ -102,6 +233,9 @@ def main():
     ap.add_argument("--rate", type=float, default=8.0)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-arrays", type=int, default=4096,
+                    help="arrays in the batched leg (0 skips it)")
+    ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device-to-device copy of the field (counter calibration)")
     args = ap.parse_args()
@@ -143,6 +277,7 @@ def main():
             dec_ms.append(s.kernel_ms()[0])
         return c, d
 
+    print(f"[main] field ready, {args.warmup} warmup + {args.steps} steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         c_bytes, d_bytes = step(False)
     torch.cuda.synchronize()
@@ -201,6 +336,10 @@ def main():
                      "algorithmic_bytes_per_launch": alg},
         "cpu_baseline": None,
     }
+
+    print(f"[main] encode {enc:.4f} ms, decode {dec:.4f} ms", file=sys.stderr, flush=True)
+    if args.batched_arrays > 0:
+        result["batched"] = batched_leg(args.batched_arrays, args.batched_steps)
 
     if not args.no_cpu_baseline:
         def gpu_words(slab):

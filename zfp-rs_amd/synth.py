@@ -104,3 +104,30 @@ def grf(shape, seed, dtype=np.float32, zero_blocks=0.0, subnormals=0.0, scale=1.
         out[sel] = vals[sel].to(out.dtype)
         del sel, vals
     return out
+
+
+def grf_tiles(seed, device, sizes=(1 << 20, 2048, 512)):
+    """One periodic k^-3 tile per dimensionality (1-D, 2-D, 3-D), float64 on
+    `device`: three FFTs in all, so that many arrays of arbitrary shape can be
+    cut from them without a transform (and a runtime FFT kernel build) per
+    shape."""
+    return {d: _grf_torch((sizes[d - 1],) * d, seed + d, device) for d in (1, 2, 3)}
+
+
+def grf_window(tiles, shape, seed, dtype=np.float32, scale=1.0):
+    """An array of `shape` cut from the tile of its dimensionality at an
+    origin drawn from `seed` (the tile is periodic, so windows wrap), scaled
+    and cast to `dtype`.  Its spectrum is the tile's, ~k^-3."""
+    import torch
+    shape = tuple(int(n) for n in shape)
+    t = tiles[len(shape)]
+    rng = np.random.default_rng(seed)
+    origin = [int(rng.integers(0, n)) for n in t.shape]
+    idx = [torch.arange(o, o + n, device=t.device) % m for o, n, m in zip(origin, shape, t.shape)]
+    w = t
+    for ax, ix in enumerate(idx):
+        w = w.index_select(ax, ix)
+    sign = -1.0 if rng.integers(2) else 1.0
+    tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32,
+           np.int64: torch.int64}[np.dtype(dtype).type]
+    return (w * (sign * scale)).to(tdt).contiguous()

@@ -375,6 +375,29 @@ def _batch(fn, streams, arrays, flags):
     return done, list(sizes)
 
 
+class Batch:
+    """Prebuilt argument tables for repeated batched calls over the same
+    streams and arrays (a serving loop's steady state)."""
+
+    def __init__(self, streams, arrays, outputs=None):
+        n = len(streams)
+        self.n = n
+        self._keep = (streams, arrays, outputs)
+        self.zs = (_SP * n)(*[s.zs for s in streams])
+        self._fin = [make_field(a) for a in arrays]
+        self.fin = (_FP * n)(*[ctypes.pointer(f) for f in self._fin])
+        outs = outputs if outputs is not None else arrays
+        self._fout = [make_field(a) for a in outs]
+        self.fout = (_FP * n)(*[ctypes.pointer(f) for f in self._fout])
+        self.sizes = (ctypes.c_size_t * n)()
+
+    def compress(self, flags=BATCH_DEVICE_FIELDS):
+        return lib().zfp_hip_compress_batch(self.zs, self.fin, self.n, self.sizes, flags)
+
+    def decompress(self, flags=BATCH_DEVICE_FIELDS):
+        return lib().zfp_hip_decompress_batch(self.zs, self.fout, self.n, self.sizes, flags)
+
+
 def compress_batch(streams, arrays, flags=0):
     """zfp_hip_compress_batch: arrays[i] into streams[i] (Stream objects), with
     one group of launches per class of arrays.  Returns (arrays coded, the
