@@ -16,7 +16,10 @@ Also reported:
                 byte for byte against the GPU's;
   batched       4096 mixed arrays (shapes, types, modes) compressed and
                 decompressed by the batched entry points: GB/s, the step's
-                p50/p99 latency, and the same work as one call per array.
+                p50/p99 latency, and the same work as one call per array;
+  serving       Poisson arrivals (1000 req/s, 16 KiB-256 MiB arrays) through
+                the continuous-batching loop and one call per request:
+                GB/s and latency p50/p99.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rate 8] [--n 512]
 """
@@ -206,6 +209,95 @@ def batched_leg(n_arrays, reps, seed=4096):
     return res
 
 
+def serving_leg(n_requests, rate, seed=2000):
+    """Poisson arrivals of compress and decompress requests (50/50), array
+    sizes log-uniform from 16 KiB to 256 MiB with the batched leg's mix of
+    dimensionality, type and mode, fields and streams resident in HBM.  The
+    same trace is replayed through the continuous-batching loop (admission by
+    an HBM budget of half the free memory) and one call per request; also
+    replayed with every request arriving at once (throughput when saturated).
+    Reports uncompressed GB/s and latency p50/p99."""
+    import math
+
+    import numpy as np
+    import torch
+
+    import serve
+    import zfp_hip as Z
+    from synth import grf_tiles, grf_window
+
+    def note(msg):
+        print(f"[serving] {msg}", file=sys.stderr, flush=True)
+
+    rng = np.random.default_rng(seed)
+    tiles = grf_tiles(seed, "cuda")
+    types = [(Z.TYPE_INT32, np.int32, 4, 2.0 ** 16), (Z.TYPE_INT64, np.int64, 8, 2.0 ** 32),
+             (Z.TYPE_FLOAT, np.float32, 4, 1.0), (Z.TYPE_DOUBLE, np.float64, 8, 1.0)]
+    trace, t = [], 0.0
+    for i in range(n_requests):
+        dims = int(rng.integers(1, 4))
+        zt, npdt, esz, scale = types[int(rng.integers(4))]
+        nbytes = math.exp(rng.uniform(math.log(16 << 10), math.log(256 << 20)))
+        side = max(3, int(round((nbytes / esz) ** (1.0 / dims))))
+        shape = tuple(max(3, int(side * rng.uniform(0.8, 1.25))) for _ in range(dims))
+        mode = int(rng.integers(3))
+        cfg = (Z.ZfpConfig.fixed_rate(float(rng.integers(4, 25)), zt, dims) if mode == 0 else
+               Z.ZfpConfig.fixed_precision(int(rng.integers(8, 49))) if mode == 1 else
+               Z.ZfpConfig.fixed_accuracy(10.0 ** rng.uniform(-9, -2)))
+        x = grf_window(tiles, shape, seed + i, dtype=npdt, scale=scale)
+        cap = Z.maximum_size(cfg, x)
+        buf = torch.zeros((cap + 7) // 8, dtype=torch.int64, device="cuda")
+        st = Z.Stream(buf, cfg)
+        t += rng.exponential(1.0 / rate)
+        if i % 2:
+            st.compress(x)  # the stream a decompress request reads
+            st.rewind()
+            req = serve.Request("decompress", st, torch.empty_like(x), x.numel() * esz, cap, t)
+        else:
+            req = serve.Request("compress", st, x, x.numel() * esz, x.numel() * esz + cap, t)
+        trace.append(req)
+    del tiles
+    torch.cuda.synchronize()
+    total = sum(r.nbytes for r in trace)
+    note(f"{n_requests} requests, {total / 1e9:.2f} GB, {rate:g} req/s")
+    budget = torch.cuda.mem_get_info()[0] // 2
+    codec = serve.GpuCodec()
+    arrivals = [r.arrival for r in trace]
+
+    def replay(fn, at_once=False):
+        for r, a in zip(trace, arrivals):
+            r.stream.rewind()
+            r.done = None
+            r.arrival = 0.0 if at_once else a
+        torch.cuda.synchronize()
+        extra = fn()
+        torch.cuda.synchronize()
+        return extra
+
+    replay(lambda: serve.serve_batched(trace, codec, budget))  # warm the workspace
+    stats = replay(lambda: serve.serve_batched(trace, codec, budget))
+    batched = serve.summary(trace)
+    batched.update(batches=stats["batches"], max_batch=stats["max_batch"])
+    note(f"batched: {batched}")
+    replay(lambda: serve.serve_one_at_a_time(trace, codec))
+    single = serve.summary(trace)
+    note(f"one call per request: {single}")
+    replay(lambda: serve.serve_batched(trace, codec, budget), at_once=True)
+    sat_b = serve.summary(trace)
+    replay(lambda: serve.serve_one_at_a_time(trace, codec), at_once=True)
+    sat_s = serve.summary(trace)
+    note(f"saturated: batched {sat_b['GBps']} GB/s, one call per request {sat_s['GBps']} GB/s")
+    for r in trace:
+        r.stream.close()
+    del trace
+    torch.cuda.empty_cache()
+    return {"rate_rps": rate, "requests": n_requests, "uncompressed_bytes": total,
+            "hbm_budget_bytes": budget, "batched": batched, "per_request": single,
+            "saturated": {"batched_GBps": sat_b["GBps"], "per_request_GBps": sat_s["GBps"],
+                          "speedup": round(sat_b["GBps"] / sat_s["GBps"], 2),
+                          "batched_p99_ms": sat_b["p99_ms"]}}
+
+
 KERNELS = {"encode": "zx::k_encode_fx<float, 3, false, true>",
            "decode": "zx::k_decode_fx<float, 3, false>"}
 
@@ -236,6 +328,9 @@ def main():
     ap.add_argument("--batched-arrays", type=int, default=4096,
                     help="arrays in the batched leg (0 skips it)")
     ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--serving-requests", type=int, default=600,
+                    help="requests in the serving leg (0 skips it)")
+    ap.add_argument("--serving-rate", type=float, default=1000.0)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device-to-device copy of the field (counter calibration)")
     args = ap.parse_args()
@@ -340,6 +435,8 @@ def main():
     print(f"[main] encode {enc:.4f} ms, decode {dec:.4f} ms", file=sys.stderr, flush=True)
     if args.batched_arrays > 0:
         result["batched"] = batched_leg(args.batched_arrays, args.batched_steps)
+    if args.serving_requests > 0:
+        result["serving"] = serving_leg(args.serving_requests, args.serving_rate)
 
     if not args.no_cpu_baseline:
         def gpu_words(slab):

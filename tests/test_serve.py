@@ -1,0 +1,86 @@
+"""The continuous-batching loop's scheduling (zfp-rs_amd/serve.py), on the CPU
+with a simulated clock and codec: admission under the HBM budget, requests
+arriving during a batch joining the next one, every request completed once."""
+import collections
+
+import serve
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += dt
+
+
+class FakeCodec:
+    """Each batch takes 1 ms plus 1 ms per MB; one call per request takes
+    0.2 ms of overhead plus the same per-byte time."""
+
+    def __init__(self, clock):
+        self.clock = clock
+        self.batches = []
+
+    def run(self, compress, decompress):
+        reqs = compress + decompress
+        self.batches.append([id(r) for r in reqs])
+        self.clock.t += 1e-3 + sum(r.nbytes for r in reqs) / 1e9
+        for r in reqs:
+            r.size = r.nbytes // 4
+
+    def run_one(self, r):
+        self.clock.t += 2e-4 + r.nbytes / 1e9
+        r.size = r.nbytes // 4
+
+
+def make_trace(n, gap, nbytes=1_000_000):
+    return [serve.Request("compress" if i % 2 else "decompress", None, None, nbytes, 2 * nbytes,
+                          i * gap) for i in range(n)]
+
+
+def test_admit_fills_the_budget_in_order():
+    reqs = collections.deque(make_trace(5, 0.0))
+    batch, used = serve.admit(reqs, budget=5_000_000)
+    assert len(batch) == 2 and used == 4_000_000 and len(reqs) == 3
+    big = collections.deque([serve.Request("compress", None, None, 10, 10 ** 12, 0.0)])
+    batch, used = serve.admit(big, budget=1)
+    assert len(batch) == 1  # a request larger than the budget runs alone
+
+
+def test_requests_arriving_during_a_batch_join_the_next():
+    clock = FakeClock()
+    codec = FakeCodec(clock)
+    trace = make_trace(40, gap=1e-4)  # 10k requests/s, 1 MB each
+    stats = serve.serve_batched(trace, codec, budget=10 ** 12, clock=clock, sleep=clock.sleep)
+    seen = [i for b in codec.batches for i in b]
+    assert sorted(seen) == sorted(id(r) for r in trace)  # each request exactly once
+    assert stats["batches"] == len(codec.batches) < 40
+    assert all(r.done >= r.arrival for r in trace)
+    # arrival order is kept across batches: each batch holds later arrivals
+    order = {id(r): k for k, r in enumerate(trace)}
+    spans = [(min(order[i] for i in b), max(order[i] for i in b)) for b in codec.batches]
+    assert all(a[1] < b[0] for a, b in zip(spans, spans[1:]))
+    s = serve.summary(trace)
+    assert s["requests"] == 40 and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_budget_bounds_the_admitted_bytes():
+    clock = FakeClock()
+    codec = FakeCodec(clock)
+    trace = make_trace(30, gap=0.0)
+    stats = serve.serve_batched(trace, codec, budget=6_000_000, clock=clock, sleep=clock.sleep)
+    assert stats["peak_bytes"] <= 6_000_000 and stats["max_batch"] == 3
+    assert stats["batches"] == 10
+
+
+def test_one_at_a_time_is_fifo():
+    clock = FakeClock()
+    codec = FakeCodec(clock)
+    trace = make_trace(10, gap=5e-3)
+    serve.serve_one_at_a_time(trace, codec, clock=clock, sleep=clock.sleep)
+    dones = [r.done for r in trace]
+    assert dones == sorted(dones) and all(r.done > r.arrival for r in trace)
