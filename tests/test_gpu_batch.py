@@ -175,3 +175,33 @@ def test_batch_of_fixed_rate_f32_cubes(rate):
         assert sizes[k] == n and np.array_equal(words_of(bufs[k], n), w)
     for s in streams:
         s.close()
+
+
+@pytest.mark.parametrize("rate,start", [(8, 0), (5, 13), (16, 64)])
+def test_batch_with_large_fixed_rate_arrays(rate, start):
+    """Large fixed-rate arrays in a batch take launches of their own (the
+    LDS-staged kernels); their bytes and values match single calls."""
+    oc = O.fixed_rate(rate, O.FLOAT, 3)
+    shapes = [(64, 128, 128), (5, 6, 7), (72, 100, 70), (9, 9, 9)]
+    arrays = [smooth(sh, np.float32, 300 + k, scale=10.0) for k, sh in enumerate(shapes)]
+    streams, bufs = [], []
+    for arr in arrays:
+        s, buf = new_stream(arr, oc, start)
+        streams.append(s)
+        bufs.append(buf)
+    src = [torch.from_numpy(a).cuda() for a in arrays]
+    done, sizes = Z.compress_batch(streams, src, Z.BATCH_DEVICE_FIELDS)
+    assert done == len(arrays)
+    outs = [torch.zeros_like(t) for t in src]
+    for s in streams:
+        s.rewind()
+        s.rseek(start)
+    ddone, dsizes = Z.decompress_batch(streams, outs, Z.BATCH_DEVICE_FIELDS)
+    assert ddone == len(arrays)
+    for k, arr in enumerate(arrays):
+        n, w, m, out = single_round_trip(arr, oc, start)
+        assert sizes[k] == n and dsizes[k] == m, k
+        assert np.array_equal(words_of(bufs[k], n), w), k
+        assert np.array_equal(outs[k].cpu().numpy().view(np.uint32), out.view(np.uint32)), k
+    for s in streams:
+        s.close()

@@ -984,10 +984,17 @@ const BEncFn kBEnc[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
 const BDecFn kBDec[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
 #undef ZX_ROW
 
+// Fixed-rate arrays of at least this many blocks get launches of their own
+// (the LDS-staged kernels of the single-array path) inside a batch; smaller
+// ones share their class's launch.
+constexpr uint64_t kSoloBlocks = 1 << 14;
+constexpr int kSoloClass = 1 << 20;
+
 // One array of a batch, on the host side.
 struct BatchItem {
   size_t input;      // position in the caller's arrays
-  int cls;           // ((type, dims, rev) variant) * 2 + variable placement
+  int cls;           // ((type, dims, rev) variant) * 2 + variable placement, or
+                     // kSoloClass + variant for an array launched on its own
   Plan plan;
   zx::BatchJob job;
   uint64_t w0;       // first stream word of the array
@@ -1106,8 +1113,8 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
         J.index = s->index;
       }
     }
-    it.cls = (((int)p.type - (int)zfp_type_int32) * 6 + (p.dims - 1) * 2 + (rev ? 1 : 0)) * 2 +
-             (fixed ? 0 : 1);
+    const int variant = ((int)p.type - (int)zfp_type_int32) * 6 + (p.dims - 1) * 2 + (rev ? 1 : 0);
+    it.cls = fixed && p.nblocks >= kSoloBlocks ? kSoloClass + variant : variant * 2 + (fixed ? 0 : 1);
     items.push_back(it);
   }
 
@@ -1138,8 +1145,22 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
     Workspace &W = ws();
     BatchBuffers &B = batch_buffers();
     std::lock_guard<std::mutex> lock(W.mu);
+    // chunks of the per-array helpers: word fills and index copies
+    std::vector<uint64_t> fill_first(n), blk_first(n);
+    uint64_t fill_chunks = 0, blk_chunks = 0;
+    for (size_t k = 0; k < n; k++) {
+      const zx::BatchJob &J = items[k].job;
+      fill_first[k] = fill_chunks;
+      blk_first[k] = blk_chunks;
+      if (encode && J.fill) {
+        uint64_t words = J.nwords;
+        if (J.fixed) words = std::min<uint64_t>(words, (J.start + items[k].plan.nblocks * J.fixed + 63) / 64);
+        fill_chunks += zx_batch_chunks(words);
+      }
+      if (!J.fixed) blk_chunks += zx_batch_chunks(items[k].plan.nblocks);
+    }
     const size_t jbytes = (n * sizeof(zx::BatchJob) + 15) / 16 * 16;
-    const size_t fbytes = n * 8;
+    const size_t fbytes = n * 8 * 3;  // firsts, fill_first, blk_first
     const size_t rbytes = n * 16;
     bool good = grow_pinned(B.hbuf, B.hcap, std::max(jbytes + fbytes, rbytes)) &&
                 Workspace::grow(B.dbuf, B.dcap, jbytes + fbytes + rbytes);
@@ -1155,9 +1176,13 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
         items[k].job.result = dres + 2 * k;
         hjobs[k] = items[k].job;
       }
-      memcpy((char *)B.hbuf + jbytes, firsts.data(), fbytes);
+      memcpy((char *)B.hbuf + jbytes, firsts.data(), n * 8);
+      memcpy((char *)B.hbuf + jbytes + n * 8, fill_first.data(), n * 8);
+      memcpy((char *)B.hbuf + jbytes + n * 16, blk_first.data(), n * 8);
       const zx::BatchJob *djobs = (const zx::BatchJob *)B.dbuf;
       const uint64_t *dfirsts = (const uint64_t *)((char *)B.dbuf + jbytes);
+      const uint64_t *dfill_first = dfirsts + n;
+      const uint64_t *dblk_first = dfirsts + 2 * n;
       uint32_t *lens = (uint32_t *)W.lens;
       uint64_t *offs = (uint64_t *)W.offs;
       auto run = [&](bool var_only, bool count) -> bool {
@@ -1166,7 +1191,48 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
           uint64_t nb = 0;
           while (e < n && items[e].cls == items[k].cls) nb += items[e++].plan.nblocks;
           const int cls = items[k].cls;
-          if (!var_only || (cls & 1)) {
+          if (cls >= kSoloClass) {
+            if (!var_only) {
+              // arrays launched on their own, through the single-array kernels
+              for (size_t q = k; q < e; q++) {
+                const zx::BatchJob &J = items[q].job;
+                const int v = cls - kSoloClass;
+                const zfp_type t = (zfp_type)(zfp_type_int32 + v / 6);
+                const int dims = (v % 6) / 2 + 1;
+                const bool rev = v & 1;
+                hipError_t r;
+                if (encode) {
+                  zx::EncodeArgs ea;
+                  ea.field = J.field;
+                  ea.cfg = J.cfg;
+                  ea.words = J.words;
+                  ea.nwords = J.nwords;
+                  ea.start = J.start;
+                  ea.fixed = J.fixed;
+                  ea.offsets = nullptr;
+                  ea.lengths = nullptr;
+                  ea.vec = J.vec;
+                  r = launch_encode(t, dims, rev, ea, st);
+                } else {
+                  zx::DecodeArgs da;
+                  da.field = J.field;
+                  da.cfg = J.cfg;
+                  da.words = J.words;
+                  da.nwords = J.nwords;
+                  da.start = J.start;
+                  da.fixed = J.fixed;
+                  da.offsets = nullptr;
+                  da.maxword = dres + 2 * q + 1;
+                  da.vec = J.vec;
+                  r = launch_decode(t, dims, rev, da, nullptr, st);
+                }
+                if (r != hipSuccess) {
+                  (void)hipGetLastError();
+                  return false;
+                }
+              }
+            }
+          } else if (!var_only || (cls & 1)) {
             zx::BatchArgs a;
             a.jobs = djobs + k;
             a.firsts = dfirsts + k;
@@ -1188,12 +1254,16 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
       good = hipMemcpyAsync(B.dbuf, B.hbuf, jbytes + fbytes, hipMemcpyHostToDevice, st) == hipSuccess;
       if (good && var_total) {
         if (encode) good = run(true, true);
-        else good = zx_batch_gather_index(djobs, (uint32_t)n, lens, st) == hipSuccess && run(true, true);
+        else
+          good = zx_batch_gather_index(djobs, dblk_first, (uint32_t)n, blk_chunks, lens, st) == hipSuccess &&
+                 run(true, true);
         good = good && zx_launch_scan(lens, var_total, 0, offs, (uint64_t *)W.tiles, st) == hipSuccess;
       }
-      good = good && zx_batch_ends(djobs, (uint32_t)n, offs, encode, st) == hipSuccess;
+      good = good && zx_batch_ends(djobs, (uint32_t)n, offs, st) == hipSuccess;
+      if (good && encode) good = zx_batch_fill(djobs, dfill_first, (uint32_t)n, fill_chunks, st) == hipSuccess;
       good = good && run(false, false);
-      if (good && encode && var_total) good = zx_batch_keep_index(djobs, (uint32_t)n, lens, st) == hipSuccess;
+      if (good && encode && var_total)
+        good = zx_batch_keep_index(djobs, dblk_first, (uint32_t)n, blk_chunks, lens, st) == hipSuccess;
       good = good && hipMemcpyAsync(B.hbuf, dres, rbytes, hipMemcpyDeviceToHost, st) == hipSuccess;
       good = good && hipStreamSynchronize(st) == hipSuccess;
       if (!good) (void)hipGetLastError();

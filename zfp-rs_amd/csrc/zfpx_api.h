@@ -90,12 +90,20 @@ hipError_t zx_launch_scan(const uint32_t *lengths, uint64_t n, uint64_t start, u
 // words[0] = first, words[1..n) = 0
 hipError_t zx_launch_fill(uint64_t *words, uint64_t n, uint64_t first, hipStream_t s);
 
-// Batch helpers over all `n` arrays of a batch (one workgroup per array):
-// copy each variable-rate array's offset index into the batch-wide lengths;
-hipError_t zx_batch_gather_index(const zx::BatchJob *jobs, uint32_t n, uint32_t *lengths, hipStream_t s);
-// write each array's end bit to result[0] and, for encode, zero its words
-// (keeping the prefix bits of the first) where `fill` is set;
-hipError_t zx_batch_ends(const zx::BatchJob *jobs, uint32_t n, const uint64_t *offsets, bool encode,
+// Batch helpers over all `n` arrays of a batch.  Per-array work is cut into
+// chunks of zx_batch_chunks(items) workgroups; first[j] is array j's first
+// chunk (an exclusive prefix over the arrays) and nchunks the total.
+uint64_t zx_batch_chunks(uint64_t items);
+// copy each variable-rate array's offset index into the batch-wide lengths
+// (chunks over blocks);
+hipError_t zx_batch_gather_index(const zx::BatchJob *jobs, const uint64_t *first, uint32_t n,
+                                 uint64_t nchunks, uint32_t *lengths, hipStream_t s);
+// write each array's end bit to result[0];
+hipError_t zx_batch_ends(const zx::BatchJob *jobs, uint32_t n, const uint64_t *offsets, hipStream_t s);
+// zero the words of the arrays with `fill` set, keeping the prefix bits of the
+// first (chunks over each array's word capacity; after zx_batch_ends);
+hipError_t zx_batch_fill(const zx::BatchJob *jobs, const uint64_t *first, uint32_t n, uint64_t nchunks,
                          hipStream_t s);
-// copy each variable-rate array's lengths to its index_out.
-hipError_t zx_batch_keep_index(const zx::BatchJob *jobs, uint32_t n, const uint32_t *lengths, hipStream_t s);
+// copy each variable-rate array's lengths to its index_out (chunks over blocks).
+hipError_t zx_batch_keep_index(const zx::BatchJob *jobs, const uint64_t *first, uint32_t n,
+                               uint64_t nchunks, const uint32_t *lengths, hipStream_t s);

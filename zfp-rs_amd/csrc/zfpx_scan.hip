@@ -80,34 +80,67 @@ __global__ void k_fill(uint64_t *w, uint64_t n, uint64_t first)
 }
 
 // ------------------------------------------------------------------ batches
-// One workgroup per array of a batch.
+// Per-array work is cut into chunks of kChunk items (words or blocks) spread
+// over the whole grid: first[j] is array j's first chunk, and a workgroup
+// finds its array by binary search.
 
-__global__ __launch_bounds__(kT) void k_batch_gather_index(const zx::BatchJob *jobs, uint32_t *lengths)
+constexpr uint64_t kChunk = 8192;
+
+__device__ uint32_t chunk_job(const uint64_t *first, uint32_t n, uint64_t c)
 {
-  const zx::BatchJob &J = jobs[blockIdx.x];
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kT) void k_batch_gather_index(const zx::BatchJob *jobs, const uint64_t *first,
+                                                           uint32_t n, uint32_t *lengths)
+{
+  const uint32_t j = chunk_job(first, n, blockIdx.x);
+  const zx::BatchJob &J = jobs[j];
   if (J.fixed || !J.index) return;
-  for (uint64_t b = threadIdx.x; b < J.field.nblocks; b += kT) lengths[J.lbase + b] = J.index[b];
+  const uint64_t b0 = (blockIdx.x - first[j]) * kChunk;
+  const uint64_t b1 = b0 + kChunk < J.field.nblocks ? b0 + kChunk : J.field.nblocks;
+  for (uint64_t b = b0 + threadIdx.x; b < b1; b += kT) lengths[J.lbase + b] = J.index[b];
 }
 
-__global__ __launch_bounds__(kT) void k_batch_ends(const zx::BatchJob *jobs, const uint64_t *offsets,
-                                                   bool encode)
+__global__ void k_batch_ends(const zx::BatchJob *jobs, uint32_t n, const uint64_t *offsets)
 {
-  const zx::BatchJob &J = jobs[blockIdx.x];
-  const uint64_t n = J.field.nblocks;
-  const uint64_t end = J.fixed ? J.start + n * (uint64_t)J.fixed
-                               : J.start + (offsets[J.lbase + n] - offsets[J.lbase]);
-  if (threadIdx.x == 0) J.result[0] = end;
-  if (!encode || !J.fill) return;
-  uint64_t nw = (end + 63) / 64;
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const zx::BatchJob &J = jobs[j];
+  const uint64_t nb = J.field.nblocks;
+  J.result[0] = J.fixed ? J.start + nb * (uint64_t)J.fixed
+                        : J.start + (offsets[J.lbase + nb] - offsets[J.lbase]);
+}
+
+// zeroes an array's words through its last block's word, keeping the prefix
+// bits of the first (after k_batch_ends)
+__global__ __launch_bounds__(kT) void k_batch_fill(const zx::BatchJob *jobs, const uint64_t *first, uint32_t n)
+{
+  const uint32_t j = chunk_job(first, n, blockIdx.x);
+  const zx::BatchJob &J = jobs[j];
+  if (!J.fill) return;
+  uint64_t nw = (J.result[0] + 63) / 64;
   if (nw > J.nwords) nw = J.nwords;
-  for (uint64_t i = threadIdx.x; i < nw; i += kT) J.words[i] = i == 0 ? J.prefix : 0ull;
+  const uint64_t w0 = (blockIdx.x - first[j]) * kChunk;
+  const uint64_t w1 = w0 + kChunk < nw ? w0 + kChunk : nw;
+  for (uint64_t i = w0 + threadIdx.x; i < w1; i += kT) J.words[i] = i == 0 ? J.prefix : 0ull;
 }
 
-__global__ __launch_bounds__(kT) void k_batch_keep_index(const zx::BatchJob *jobs, const uint32_t *lengths)
+__global__ __launch_bounds__(kT) void k_batch_keep_index(const zx::BatchJob *jobs, const uint64_t *first,
+                                                         uint32_t n, const uint32_t *lengths)
 {
-  const zx::BatchJob &J = jobs[blockIdx.x];
+  const uint32_t j = chunk_job(first, n, blockIdx.x);
+  const zx::BatchJob &J = jobs[j];
   if (J.fixed || !J.index_out) return;
-  for (uint64_t b = threadIdx.x; b < J.field.nblocks; b += kT) J.index_out[b] = lengths[J.lbase + b];
+  const uint64_t b0 = (blockIdx.x - first[j]) * kChunk;
+  const uint64_t b1 = b0 + kChunk < J.field.nblocks ? b0 + kChunk : J.field.nblocks;
+  for (uint64_t b = b0 + threadIdx.x; b < b1; b += kT) J.index_out[b] = lengths[J.lbase + b];
 }
 
 }  // namespace
@@ -135,24 +168,35 @@ hipError_t zx_launch_fill(uint64_t *words, uint64_t n, uint64_t first, hipStream
   return hipGetLastError();
 }
 
-hipError_t zx_batch_gather_index(const zx::BatchJob *jobs, uint32_t n, uint32_t *lengths, hipStream_t s)
+uint64_t zx_batch_chunks(uint64_t items) { return (items + kChunk - 1) / kChunk; }
+
+hipError_t zx_batch_gather_index(const zx::BatchJob *jobs, const uint64_t *first, uint32_t n,
+                                 uint64_t nchunks, uint32_t *lengths, hipStream_t s)
 {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_gather_index, dim3(n), dim3(kT), 0, s, jobs, lengths);
+  if (nchunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_gather_index, dim3(nchunks), dim3(kT), 0, s, jobs, first, n, lengths);
   return hipGetLastError();
 }
 
-hipError_t zx_batch_ends(const zx::BatchJob *jobs, uint32_t n, const uint64_t *offsets, bool encode,
+hipError_t zx_batch_ends(const zx::BatchJob *jobs, uint32_t n, const uint64_t *offsets, hipStream_t s)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_ends, dim3((n + 255) / 256), dim3(256), 0, s, jobs, n, offsets);
+  return hipGetLastError();
+}
+
+hipError_t zx_batch_fill(const zx::BatchJob *jobs, const uint64_t *first, uint32_t n, uint64_t nchunks,
                          hipStream_t s)
 {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_ends, dim3(n), dim3(kT), 0, s, jobs, offsets, encode);
+  if (nchunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_fill, dim3(nchunks), dim3(kT), 0, s, jobs, first, n);
   return hipGetLastError();
 }
 
-hipError_t zx_batch_keep_index(const zx::BatchJob *jobs, uint32_t n, const uint32_t *lengths, hipStream_t s)
+hipError_t zx_batch_keep_index(const zx::BatchJob *jobs, const uint64_t *first, uint32_t n,
+                               uint64_t nchunks, const uint32_t *lengths, hipStream_t s)
 {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch_keep_index, dim3(n), dim3(kT), 0, s, jobs, lengths);
+  if (nchunks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_batch_keep_index, dim3(nchunks), dim3(kT), 0, s, jobs, first, n, lengths);
   return hipGetLastError();
 }
