@@ -19,7 +19,10 @@ Also reported:
                 p50/p99 latency, and the same work as one call per array;
   serving       Poisson arrivals (1000 req/s, 16 KiB-256 MiB arrays) through
                 the continuous-batching loop and one call per request:
-                GB/s and latency p50/p99.
+                GB/s and latency p50/p99;
+  variable_rate the variable-rate configurations (384^3 f64 accuracy and
+                precision, 8192^2 f32 precision, 256^3 i32 reversible):
+                compress and decompress GB/s through the offset index.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rate 8] [--n 512]
 """
@@ -298,6 +301,71 @@ def serving_leg(n_requests, rate, seed=2000):
                           "batched_p99_ms": sat_b["p99_ms"]}}
 
 
+def variable_rate_leg(steps=5):
+    """The variable-rate configurations: 384^3 f64 k^-3 field scaled into
+    [-1e3, 1e3] at fixed accuracy 1e-6 and fixed precision 32; 8192^2 f32 at
+    fixed precision 20; 256^3 i32 random walk in reversible mode.  Compress
+    (length pass, scan, write pass; the offset index kept) and decompress
+    through the index; best step of `steps`; round trip checked (reversible
+    exact, fixed accuracy within the tolerance)."""
+    import numpy as np
+    import torch
+
+    import zfp_hip as Z
+    from synth import grf
+
+    def note(msg):
+        print(f"[variable] {msg}", file=sys.stderr, flush=True)
+
+    cases = []
+    v1 = (grf((384, 384, 384), seed=11, dtype=np.float64, device="cuda") * 300.0).clamp_(-1e3, 1e3)
+    cases.append(("384^3 f64 fixed-accuracy 1e-6", v1, Z.ZfpConfig.fixed_accuracy(1e-6), 1e-6))
+    cases.append(("384^3 f64 fixed-precision 32", v1, Z.ZfpConfig.fixed_precision(32), None))
+    v2 = grf((8192, 8192), seed=12, dtype=np.float32, device="cuda")
+    cases.append(("8192^2 f32 fixed-precision 20", v2, Z.ZfpConfig.fixed_precision(20), None))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(13)
+    v3 = torch.randint(-3, 4, (256 ** 3,), generator=g, device="cuda", dtype=torch.int32)
+    v3 = torch.cumsum(v3, 0, dtype=torch.int32).reshape(256, 256, 256)
+    cases.append(("256^3 i32 random walk reversible", v3, Z.ZfpConfig.reversible(), 0))
+    out = []
+    for name, x, cfg, tol in cases:
+        cap = Z.maximum_size(cfg, x)
+        buf = torch.zeros((cap + 7) // 8, dtype=torch.int64, device="cuda")
+        y = torch.empty_like(x)
+        s = Z.Stream(buf, cfg)
+        best_c = best_d = None
+        for _ in range(steps + 1):
+            s.rewind()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            n = s.compress(x)
+            t1 = time.perf_counter()
+            s.rewind()
+            s.decompress(y)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            best_c = t1 - t0 if best_c is None else min(best_c, t1 - t0)
+            best_d = t2 - t1 if best_d is None else min(best_d, t2 - t1)
+        nb = x.numel() * x.element_size()
+        if tol == 0:
+            ok = bool(torch.equal(x, y))
+        elif tol is not None:
+            ok = bool((x - y).abs().max().item() <= tol)
+        else:
+            ok = True
+        s.close()
+        r = {"config": name, "uncompressed_bytes": nb, "stream_bytes": n,
+             "ratio": round(nb / n, 3), "compress_GBps": round(nb / best_c / 1e9, 2),
+             "decompress_GBps": round(nb / best_d / 1e9, 2), "round_trip_ok": ok}
+        note(str(r))
+        out.append(r)
+        del buf, y
+    del v1, v2, v3, cases
+    torch.cuda.empty_cache()
+    return out
+
+
 KERNELS = {"encode": "zx::k_encode_fx<float, 3, false, true>",
            "decode": "zx::k_decode_fx<float, 3, false>"}
 
@@ -331,6 +399,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=600,
                     help="requests in the serving leg (0 skips it)")
     ap.add_argument("--serving-rate", type=float, default=1000.0)
+    ap.add_argument("--no-variable-rate", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device-to-device copy of the field (counter calibration)")
     args = ap.parse_args()
@@ -437,6 +506,8 @@ def main():
         result["batched"] = batched_leg(args.batched_arrays, args.batched_steps)
     if args.serving_requests > 0:
         result["serving"] = serving_leg(args.serving_requests, args.serving_rate)
+    if not args.no_variable_rate:
+        result["variable_rate"] = variable_rate_leg()
 
     if not args.no_cpu_baseline:
         def gpu_words(slab):
