@@ -572,6 +572,37 @@ struct Source {
     return v & lowmask(n);
   }
   ZX_HD uint64_t pos() const { return idx * 64 - bits; }
+  ZX_HD uint64_t words_loaded() const { return idx; }
+};
+
+// Over a staged copy in LDS the reader keeps only its bit position: each peek
+// loads the two words under it, so a step is one add and no refill branch.
+// words_loaded() is the reference cursor's word_pos, ceil(pos / 64).
+template <>
+struct Source<StagedWords> {
+  StagedWords mem;
+  uint32_t p;
+
+  ZX_HD void init(const StagedWords &m, uint64_t bitpos)
+  {
+    mem = m;
+    p = (uint32_t)bitpos;
+  }
+  ZX_HD uint64_t peek() const
+  {
+    const uint32_t i = p >> 6, s = p & 63;
+    const uint64_t lo = mem(i), hi = mem(i + 1);
+    return (lo >> s) | ((hi << 1) << (63 - s));
+  }
+  ZX_HD void consume(uint32_t n) { p += n; }
+  ZX_HD uint64_t read(uint32_t n)
+  {
+    const uint64_t v = peek();
+    consume(n);
+    return v & lowmask(n);
+  }
+  ZX_HD uint64_t pos() const { return p; }
+  ZX_HD uint64_t words_loaded() const { return (p + 63) >> 6; }
 };
 
 typedef Source<GlobalWords> WordSource;
@@ -699,40 +730,36 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
     PW pl[16];
     ZX_UNROLL
     for (int i = 15; i >= 0; i--) {
+      // a plane is coded while budget remains and k > kmin; stopping at
+      // kmin still takes the plane off k (round_state's precision)
+      const bool live = act && bits != 0;
+      k -= live ? 1u : 0u;  // == 16c + i when live
+      m = live ? 0u : m;
+      act = live && k >= kmin;
       pl[i] = 0;
       if (act) {
-        if (bits == 0) {
-          act = false;
-        } else {
-          m = 0;
-          k -= 1;  // == 16c + i
-          if (k < kmin) {
-            act = false;
-          } else {
-            m = umin(n, bits);
-            bits -= m;
-            uint64_t x = r.read(m);
-            // group tests, one one-bit (or the final negative test) per
-            // step, written without branches so the wave's lanes stay in step
-            bool go = bits != 0 && n < (uint32_t)SIZE;
-            while (go) {
-              const uint64_t ahead = r.peek();
-              const bool one = (ahead & 1u) != 0;
-              bits -= 1;
-              const uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
-              const uint32_t zeros = ctz64(ahead >> 1);
-              const bool hit = zeros < limit;
-              const uint32_t adv = hit ? zeros : limit;
-              r.consume(one ? adv + 1 + (hit ? 1u : 0u) : 1u);
-              bits -= one ? adv + (hit ? 1u : 0u) : 0u;
-              x |= one ? 1ull << (n + adv) : 0ull;
-              n += one ? adv + 1 : 0u;
-              m = one ? n : (uint32_t)SIZE;
-              go = one && bits != 0 && n < (uint32_t)SIZE;
-            }
-            pl[i] = (PW)x;
-          }
+        m = umin(n, bits);
+        bits -= m;
+        uint64_t x = r.read(m);
+        // group tests, one one-bit (or the final negative test) per
+        // step, written without branches so the wave's lanes stay in step
+        bool go = bits != 0 && n < (uint32_t)SIZE;
+        while (go) {
+          const uint64_t ahead = r.peek();
+          const bool one = (ahead & 1u) != 0;
+          bits -= 1;
+          const uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
+          const uint32_t zeros = ctz64(ahead >> 1);
+          const bool hit = zeros < limit;
+          const uint32_t adv = hit ? zeros : limit;
+          r.consume(one ? adv + 1 + (hit ? 1u : 0u) : 1u);
+          bits -= one ? adv + (hit ? 1u : 0u) : 0u;
+          x |= one ? 1ull << (n + adv) : 0ull;
+          n += one ? adv + 1 : 0u;
+          m = one ? n : (uint32_t)SIZE;
+          go = one && bits != 0 && n < (uint32_t)SIZE;
         }
+        pl[i] = (PW)x;
       }
     }
     PW any = 0;

@@ -96,7 +96,7 @@ template <class Src>
 ZX_HD uint64_t loaded_words(const Src &r, uint64_t pos, uint32_t minbits)
 {
   const uint64_t used = r.pos() - pos;
-  uint64_t loaded = r.idx;
+  uint64_t loaded = r.words_loaded();
   if (used < minbits) {
     const uint64_t target = pos + minbits;
     if (target % 64) loaded = loaded > target / 64 + 1 ? loaded : target / 64 + 1;
