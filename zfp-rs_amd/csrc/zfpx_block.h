@@ -273,31 +273,31 @@ ZX_HD void tr_stage(uint32_t (&a)[N])
   }
 }
 
-// rows 16..31 (HI) or 0..15 (!HI) of the transpose of a[32]
-template <bool HI>
-ZX_HD void tr32_half(const uint32_t (&a)[32], uint32_t (&o)[16])
-{
-  ZX_UNROLL
-  for (int i = 0; i < 16; i++)
-    o[i] = HI ? (a[i] >> 16) | (a[i + 16] & 0xffff0000u) : (a[i] & 0x0000ffffu) | (a[i + 16] << 16);
-  tr_stage<8>(o);
-  tr_stage<4>(o);
-  tr_stage<2>(o);
-  tr_stage<1>(o);
-}
-
 // Plane word: one bit per coefficient.
 template <int SIZE> struct PWord { typedef uint32_t T; };
 template <> struct PWord<64> { typedef uint64_t T; };
 
-// Planes 16c..16c+15 of a block: out[i] bit j = bit 16c+i of coefficient j.
-template <int P, int SIZE>
+// Planes are handled CH at a time (a chunk).  The decoder keeps a chunk of
+// decoded planes live beside the coefficients, and smaller chunks help it fit
+// four waves per SIMD; the encoder transposes fewer, larger chunks.
+constexpr int kEncodeChunk = 16;
+constexpr int kDecodeChunk = 8;
+
+// Planes CH*c .. CH*c+CH-1 of a block (CH = 8 or 16): out[i] bit j = bit
+// CH*c+i of coefficient j.  These are rows of the transpose of the 32 x 32 bit
+// matrix of each 32 coefficients (row r = coefficient r); the stages that
+// separate rows are only computed for the half or quarter needed.
+template <int P, int SIZE, int CH>
 ZX_HD void plane_chunk(const typename Ints<P>::U (&u)[SIZE], int c,
-                       typename PWord<SIZE>::T (&out)[16])
+                       typename PWord<SIZE>::T (&out)[CH])
 {
+  static_assert(CH == 8 || CH == 16, "chunk of 8 or 16 planes");
   typedef typename PWord<SIZE>::T W;
   constexpr int G = SIZE > 32 ? 2 : 1;
-  const int h = c >> 1;  // 32-bit group of the coefficients
+  const int first = CH * c;
+  const int h = first >> 5;            // 32-bit half of the coefficients
+  const bool hi16 = (first & 16) != 0;  // rows 16..31 of the half
+  const bool hi8 = (first & 8) != 0;    // rows 8..15 of those (CH == 8)
   ZX_UNROLL
   for (int g = 0; g < G; g++) {
     uint32_t a[32];
@@ -307,40 +307,60 @@ ZX_HD void plane_chunk(const typename Ints<P>::U (&u)[SIZE], int c,
       a[r] = i < SIZE ? (uint32_t)((uint64_t)u[i < SIZE ? i : 0] >> (32 * h)) : 0u;
     }
     uint32_t o[16];
-    if (c & 1) tr32_half<true>(a, o);
-    else tr32_half<false>(a, o);
     ZX_UNROLL
-    for (int i = 0; i < 16; i++) {
-      if (g == 0) out[i] = (W)o[i];
-      else out[i] |= (W)((uint64_t)o[i] << 32);
+    for (int i = 0; i < 16; i++)
+      o[i] = hi16 ? (a[i] >> 16) | (a[i + 16] & 0xffff0000u) : (a[i] & 0x0000ffffu) | (a[i + 16] << 16);
+    uint32_t q[CH];
+    if constexpr (CH == 16) {
+      tr_stage<8>(o);
+      ZX_UNROLL
+      for (int i = 0; i < 16; i++) q[i] = o[i];
+    } else {
+      ZX_UNROLL
+      for (int i = 0; i < 8; i++)
+        q[i] = hi8 ? ((o[i] >> 8) & 0x00ff00ffu) | (o[i + 8] & 0xff00ff00u)
+                   : (o[i] & 0x00ff00ffu) | ((o[i + 8] << 8) & 0xff00ff00u);
+    }
+    tr_stage<4>(q);
+    tr_stage<2>(q);
+    tr_stage<1>(q);
+    ZX_UNROLL
+    for (int i = 0; i < CH; i++) {
+      if (g == 0) out[i] = (W)q[i];
+      else out[i] |= (W)((uint64_t)q[i] << 32);
     }
   }
 }
 
-// ORs the decoded planes 16c..16c+15 (pl[i] bit j = bit 16c+i of coefficient
-// j) into the coefficients: a 16 x 32 bit-matrix transpose per 32 coefficients.
-template <int P, int SIZE>
-ZX_HD void chunk_to_coeffs(const typename PWord<SIZE>::T (&pl)[16], int c,
+// ORs the decoded planes CH*c .. CH*c+CH-1 (pl[i] bit j = bit CH*c+i of
+// coefficient j) into the coefficients: per 32 coefficients, the CH plane
+// words are the top rows of an otherwise zero 32 x 32 bit matrix, and the
+// stages below CH leave coefficient CH*m + i in field m (CH bits wide) of
+// word i.
+template <int P, int SIZE, int CH>
+ZX_HD void chunk_to_coeffs(const typename PWord<SIZE>::T (&pl)[CH], int c,
                            typename Ints<P>::U (&u)[SIZE])
 {
+  static_assert(CH == 8 || CH == 16, "chunk of 8 or 16 planes");
   typedef typename Ints<P>::U U;
   constexpr int G = SIZE > 32 ? 2 : 1;
+  constexpr uint32_t FM = CH == 16 ? 0xffffu : 0xffu;
   ZX_UNROLL
   for (int g = 0; g < G; g++) {
-    uint32_t t[16];
+    uint32_t t[CH];
     ZX_UNROLL
-    for (int i = 0; i < 16; i++) t[i] = (uint32_t)((uint64_t)pl[i] >> (32 * g));
-    // rows 16..31 of a 32 x 32 transpose whose rows 0..15 are zero: after the
-    // stages, t[i] holds coefficient i in its low half and i + 16 in its high
-    tr_stage<8>(t);
+    for (int i = 0; i < CH; i++) t[i] = (uint32_t)((uint64_t)pl[i] >> (32 * g));
+    if constexpr (CH == 16) tr_stage<8>(t);
     tr_stage<4>(t);
     tr_stage<2>(t);
     tr_stage<1>(t);
     ZX_UNROLL
-    for (int i = 0; i < 16; i++) {
-      const int j0 = 32 * g + i, j1 = 32 * g + i + 16;
-      if (j0 < SIZE) u[j0 < SIZE ? j0 : 0] |= (U)(t[i] & 0xffffu) << (16 * c);
-      if (j1 < SIZE) u[j1 < SIZE ? j1 : 0] |= (U)(t[i] >> 16) << (16 * c);
+    for (int i = 0; i < CH; i++) {
+      ZX_UNROLL
+      for (int f = 0; f < 32 / CH; f++) {
+        const int j = 32 * g + CH * f + i;
+        if (j < SIZE) u[j < SIZE ? j : 0] |= (U)((t[i] >> (CH * f)) & FM) << (CH * c);
+      }
     }
   }
 }
@@ -680,15 +700,16 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   bits -= empty;
   const uint32_t kstart = (uint32_t)P - empty;  // planes below kstart remain
   uint32_t n = 0;
+  constexpr int CH = kEncodeChunk;
   ZX_UNROLL
-  for (int c = P / 16 - 1; c >= 0; c--) {
-    const bool need = bits != 0 && kstart > (uint32_t)(16 * c) && kmin <= (uint32_t)(16 * c + 15);
+  for (int c = P / CH - 1; c >= 0; c--) {
+    const bool need = bits != 0 && kstart > (uint32_t)(CH * c) && kmin <= (uint32_t)(CH * c + CH - 1);
     if (!wave_any(need)) continue;
-    PW ch[16];
-    plane_chunk<P, SIZE>(u, c, ch);
+    PW ch[CH];
+    plane_chunk<P, SIZE, CH>(u, c, ch);
     ZX_UNROLL
-    for (int i = 15; i >= 0; i--) {
-      const uint32_t k = (uint32_t)(16 * c + i);
+    for (int i = CH - 1; i >= 0; i--) {
+      const uint32_t k = (uint32_t)(CH * c + i);
       if (bits != 0 && k < kstart && k >= kmin) {
         uint64_t clo, chi;
         uint32_t nn;
@@ -723,17 +744,18 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
   for (int i = 0; i < SIZE; i++) u[i] = 0;
   uint32_t bits = budget, n = 0, m = 0, k = P;
   bool act = true;
+  constexpr int CH = kDecodeChunk;
   ZX_UNROLL
-  for (int c = P / 16 - 1; c >= 0; c--) {
+  for (int c = P / CH - 1; c >= 0; c--) {
     if (!wave_any(act)) break;
-    // planes 16c..16c+15, folded into u once the chunk is decoded
-    PW pl[16];
+    // planes CH*c .. CH*c+CH-1, folded into u once the chunk is decoded
+    PW pl[CH];
     ZX_UNROLL
-    for (int i = 15; i >= 0; i--) {
+    for (int i = CH - 1; i >= 0; i--) {
       // a plane is coded while budget remains and k > kmin; stopping at
       // kmin still takes the plane off k (round_state's precision)
       const bool live = act && bits != 0;
-      k -= live ? 1u : 0u;  // == 16c + i when live
+      k -= live ? 1u : 0u;  // == CH*c + i when live
       m = live ? 0u : m;
       act = live && k >= kmin;
       pl[i] = 0;
@@ -764,8 +786,8 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
     }
     PW any = 0;
     ZX_UNROLL
-    for (int i = 0; i < 16; i++) any |= pl[i];
-    if (wave_any(any != 0)) chunk_to_coeffs<P, SIZE>(pl, c, u);
+    for (int i = 0; i < CH; i++) any |= pl[i];
+    if (wave_any(any != 0)) chunk_to_coeffs<P, SIZE, CH>(pl, c, u);
   }
   if (act && bits != 0) {  // ran through plane 0 with budget left (kmin == 0)
     m = 0;

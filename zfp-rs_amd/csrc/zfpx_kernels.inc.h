@@ -173,9 +173,11 @@ __global__ __launch_bounds__(kThreads) void k_encode_fx(EncodeArgs a, FxPlan p)
     if (!fx_enc_store<OWN>(a.words, a.nwords, lds, p, q, i, a.fixed)) break;
 }
 
-// Fixed-rate decode through LDS: stage the group's words, then decode.
+// Fixed-rate decode through LDS: stage the group's words, then decode.  Held
+// to four waves per SIMD (128 VGPRs, a few spilled): the group-test loop waits
+// on LDS reads, and the extra wave hides more of that than the spills cost.
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) void k_decode_fx(DecodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
