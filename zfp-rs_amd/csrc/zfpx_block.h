@@ -253,6 +253,9 @@ ZX_HD bool wave_any(bool p)
 #endif
 }
 
+// True when p holds on every active lane of the wave.
+ZX_HD bool wave_all(bool p) { return !wave_any(!p); }
+
 // ---------------------------------------------------------------- bit planes
 
 // Stages of the 32x32 bit-matrix transpose (row r bit c <-> row c bit r).
@@ -1023,6 +1026,43 @@ template <> struct Bits<int64_t> {
   ZX_HD static int64_t make(uint64_t u) { return (int64_t)u; }
 };
 
+// fwd_cast of a whole block (encode/core.rs:323-423): q = trunc(x * 2^(P-2-emax)),
+// scaled in two exact steps (x * 2^64 first) when 2^(P-2-emax) is not a
+// normal power of two (emax < MINCAST), and the minimum integer wherever the
+// scaled value is not within +-2^(P-1) (NaN, infinity, overflow), as x86's
+// conversion gives.  The products are exact wherever they are at least one in
+// magnitude (both factors are powers of two and the first step cannot leave
+// the normal range), and smaller ones truncate to zero whatever their
+// rounding, so the hardware multiply and truncate give the exact result.
+template <class S, int SIZE>
+ZX_HD void cast_block(const typename Ints<ST<S>::P>::U (&b)[SIZE], int emax,
+                      typename Ints<ST<S>::P>::I (&q)[SIZE])
+{
+  typedef typename Ints<ST<S>::P>::I I;
+  constexpr int P = ST<S>::P;
+  const bool tiny = emax < ST<S>::MINCAST;
+  const I imin = (I)((typename Ints<P>::U)1 << (P - 1));
+  if constexpr (P == 32) {
+    const float s1 = tiny ? 0x1p64f : 1.0f;
+    const float s2 = pow2f(tiny ? 30 - 64 - emax : 30 - emax);
+    ZX_UNROLL
+    for (int i = 0; i < SIZE; i++) {
+      const float v = (u2f((uint32_t)b[i]) * s1) * s2;
+      const bool in = fabsf(v) < 0x1p31f;
+      q[i] = in ? (I)(in ? v : 0.0f) : imin;
+    }
+  } else {
+    const double s1 = tiny ? 0x1p64 : 1.0;
+    const double s2 = pow2d(tiny ? 62 - 64 - emax : 62 - emax);
+    ZX_UNROLL
+    for (int i = 0; i < SIZE; i++) {
+      const double v = (u2d((uint64_t)b[i]) * s1) * s2;
+      const bool in = fabs(v) < 0x1p63;
+      q[i] = in ? (I)(in ? v : 0.0) : imin;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- lossy blocks
 
 // Encode one lossy block of raw value bits (encode/float.rs:22-113,
@@ -1039,34 +1079,13 @@ ZX_HD uint32_t encode_lossy(Sink &w, const typename Ints<ST<S>::P>::U (&b)[Blk<D
     for (int i = 0; i < SIZE; i++) q[i] = (I)b[i];
     return encode_int_block<P, D>(w, q, c.minbits, c.maxbits, c.maxprec, c.rounding);
   } else {
-    bool special;
-    const int emax = exponent_block<S, SIZE>(b, special);
+    const int emax = exponent_block<S, SIZE>(b);
     const uint32_t prec = precision_f(emax, c.maxprec, c.minexp, D, c.tight);
     const uint32_t e = prec ? (uint32_t)(emax + ST<S>::EBIAS) : 0u;
     if (e) {
       const uint32_t hb = ST<S>::EBITS + 1;
       w.put(2ull * e + 1, hb);
-      // Without infinities or NaNs, and above the range where a subnormal
-      // value can scale to one or more, x * 2^(P-2-emax) is exact wherever
-      // it is at least one, so the hardware product and truncation give
-      // trunc(x * 2^(P-2-emax)) exactly.
-      if (!special && emax >= ST<S>::MINCAST + 1) {
-        if constexpr (P == 32) {
-          const float sc = pow2f(30 - emax);
-          ZX_UNROLL
-          for (int i = 0; i < SIZE; i++) q[i] = (I)(u2f((uint32_t)b[i]) * sc);
-        } else {
-          const double sc = pow2d(62 - emax);
-          ZX_UNROLL
-          for (int i = 0; i < SIZE; i++) q[i] = (I)(u2d((uint64_t)b[i]) * sc);
-        }
-      } else {
-        ZX_UNROLL
-        for (int i = 0; i < SIZE; i++) {
-          bool ex;
-          q[i] = cast_one<S>(b[i], emax, ex);
-        }
-      }
+      cast_block<S, SIZE>(b, emax, q);
       return hb + encode_int_block<P, D>(w, q, sat_sub(c.minbits, hb), sat_sub(c.maxbits, hb), prec,
                                          c.rounding);
     }
