@@ -945,38 +945,6 @@ ZX_HD typename Ints<ST<S>::P>::I cast_one(typename Ints<ST<S>::P>::U b, int emax
   return neg ? (I)(0 - mag) : (I)mag;
 }
 
-// x * 2^e for a nonzero finite float x with |x| >= 1, rounded to nearest even
-// once, subnormal results included (the product s * x of inv_cast when s is a
-// representable power of two).
-ZX_HD float ldexp_rne(float x, int e)
-{
-  uint32_t b = f2u(x);
-  uint32_t sign = b & 0x80000000u;
-  int ex = (int)((b >> 23) & 0xff) + e;
-  uint32_t man = (b & 0x7fffffu) | 0x800000u;
-  if (ex >= 255) return u2f(sign | 0x7f800000u);
-  if (ex >= 1) return u2f(sign | ((uint32_t)ex << 23) | (man & 0x7fffffu));
-  const int shift = 1 - ex;  // >= 1
-  if (shift > 25) return u2f(sign);
-  uint32_t q = man >> shift, rem = man & ((1u << shift) - 1), half = 1u << (shift - 1);
-  if (rem > half || (rem == half && (q & 1u))) q++;
-  return u2f(sign | q);
-}
-ZX_HD double ldexp_rne(double x, int e)
-{
-  uint64_t b = d2u(x);
-  uint64_t sign = b & 0x8000000000000000ull;
-  int ex = (int)((b >> 52) & 0x7ff) + e;
-  uint64_t man = (b & 0xfffffffffffffull) | 0x10000000000000ull;
-  if (ex >= 2047) return u2d(sign | 0x7ff0000000000000ull);
-  if (ex >= 1) return u2d(sign | ((uint64_t)ex << 52) | (man & 0xfffffffffffffull));
-  const int shift = 1 - ex;
-  if (shift > 54) return u2d(sign);
-  uint64_t q = man >> shift, rem = man & ((1ull << shift) - 1), half = 1ull << (shift - 1);
-  if (rem > half || (rem == half && (q & 1u))) q++;
-  return u2d(sign | q);
-}
-
 // exact round-to-nearest-even int64 -> double
 ZX_HD double i64_to_f64(int64_t i)
 {
@@ -985,27 +953,33 @@ ZX_HD double i64_to_f64(int64_t i)
   return hi + lo;
 }
 
-// inv_cast of one value: fl(fl(i) * 2^e) with e = emax - (P - 2), where the
-// scale 2^e is first rounded to the type (decode/core.rs:162-175).
-ZX_HD float inv_cast_one(int32_t i, int e)
+// 2^e in the type, subnormal scales included, zero below them (ldexp(1, e)
+// as inv_cast computes its scale).
+ZX_HD float pow2f_scale(int e)
 {
-  if (i == 0) return 0.0f;
-  const float fi = (float)i;
-  if (e >= -126) {
-    return ldexpf(fi, e);
-  }
-  if (e < -149) return u2f(f2u(fi) & 0x80000000u);  // the scale rounds to zero
-  return ldexp_rne(fi, e);
+  return e >= -126 ? pow2f(e) : e >= -149 ? u2f(1u << (e + 149)) : 0.0f;
 }
-ZX_HD double inv_cast_one(int64_t i, int e)
+ZX_HD double pow2d_scale(int e)
 {
-  if (i == 0) return 0.0;
-  const double fi = i64_to_f64(i);
-  if (e >= -1022) {
-    return ldexp(fi, e);
+  return e >= -1022 ? pow2d(e) : e >= -1074 ? u2d(1ull << (e + 1074)) : 0.0;
+}
+
+// inv_cast (decode/core.rs:162-175): x = fl(s * fl(i)) with s = 2^e rounded
+// to the type, e = emax - (P - 2): one conversion (to nearest even) and one
+// multiply, subnormal results rounded by the hardware (IEEE denormals).
+template <class S, int SIZE>
+ZX_HD void inv_cast_block(const typename Ints<ST<S>::P>::I (&q)[SIZE], int e,
+                          typename Ints<ST<S>::P>::U (&b)[SIZE])
+{
+  if constexpr (ST<S>::P == 32) {
+    const float sc = pow2f_scale(e);
+    ZX_UNROLL
+    for (int i = 0; i < SIZE; i++) b[i] = f2u(sc * (float)q[i]);
+  } else {
+    const double sc = pow2d_scale(e);
+    ZX_UNROLL
+    for (int i = 0; i < SIZE; i++) b[i] = d2u(sc * i64_to_f64(q[i]));
   }
-  if (e < -1074) return u2d(d2u(fi) & 0x8000000000000000ull);
-  return ldexp_rne(fi, e);
 }
 
 template <class S> struct Bits;
@@ -1117,8 +1091,7 @@ ZX_HD void decode_lossy(Src &r, typename Ints<ST<S>::P>::U (&b)[Blk<D>::SIZE], c
       const uint32_t prec = precision_f(emax, c.maxprec, c.minexp, D, c.tight);
       decode_int_block<P, D>(r, q, sat_sub(c.maxbits, hb), prec, c.rounding);
       const int e = emax - (P - 2);
-      ZX_UNROLL
-      for (int i = 0; i < SIZE; i++) b[i] = Bits<S>::get(inv_cast_one(q[i], e));
+      inv_cast_block<S, SIZE>(q, e, b);
     } else {
       ZX_UNROLL
       for (int i = 0; i < SIZE; i++) b[i] = 0;
@@ -1250,8 +1223,7 @@ ZX_HD void decode_reversible(Src &r, typename Ints<ST<S>::P>::U (&b)[Blk<D>::SIZ
     rev_decode_ints<P, D>(r, q, sat_sub(c.maxbits, 2 + ST<S>::EBITS));
     inv_xform<D>(q, RInvL());
     const int e = emax - (P - 2);
-    ZX_UNROLL
-    for (int i = 0; i < SIZE; i++) b[i] = Bits<S>::get(inv_cast_one(q[i], e));
+    inv_cast_block<S, SIZE>(q, e, b);
   }
   }
 }
