@@ -32,8 +32,12 @@ def specials(shape, dtype, seed):
     if np.dtype(dtype).kind == "f":
         fi = np.finfo(dtype)
         picks = [np.nan, np.inf, -np.inf, 0.0, -0.0, fi.tiny / 8, -fi.tiny / 3, fi.max, fi.tiny]
+        # signaling NaNs (quiet bit clear), both signs
+        u = np.uint32 if np.dtype(dtype).itemsize == 4 else np.uint64
+        snan = [0x7fa00000, 0xff800001] if u is np.uint32 else [0x7ff4000000000000, 0xfff0000000000001]
+        pool = np.concatenate([np.array(picks, dtype=dtype), np.array(snan, dtype=u).view(dtype)])
         idx = rng.choice(a.size, size=max(1, a.size // 10), replace=False)
-        a[idx] = rng.choice(np.array(picks, dtype=dtype), size=idx.size)
+        a[idx] = pool[rng.integers(0, pool.size, size=idx.size)]
         # a run of tiny values so whole blocks are tiny
         n = min(a.size, 64)
         a[:n] = (rng.normal(size=n) * float(fi.tiny) * 1e3).astype(dtype)

@@ -60,6 +60,8 @@ ZX_HD uint32_t ctz64(uint64_t x) { return x ? (uint32_t)__builtin_ctzll(x) : 64u
 ZX_HD uint32_t clz32(uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; }
 ZX_HD uint32_t clz64(uint64_t x) { return x ? (uint32_t)__builtin_clzll(x) : 64u; }
 ZX_HD uint64_t lowmask(uint32_t n) { return n < 64 ? (1ull << n) - 1 : ~0ull; }
+ZX_HD uint32_t popc32(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
+ZX_HD uint32_t popc64(uint64_t x) { return (uint32_t)__builtin_popcountll(x); }
 ZX_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 ZX_HD uint32_t sat_sub(uint32_t a, uint32_t b) { return a > b ? a - b : 0u; }
 
@@ -632,6 +634,96 @@ typedef Source<GlobalWords> WordSource;
 
 // ---------------------------------------------------------------- plane coder
 
+// expand8(b): the one-bits of the byte b, bit i (the r-th one) moved to i + r.
+ZX_HD constexpr uint32_t expand8(uint32_t b)
+{
+  uint32_t e = 0, r = 0;
+  for (uint32_t i = 0; i < 8; i++)
+    if ((b >> i) & 1u) e |= 1u << (i + r++);
+  return e;
+}
+
+struct Expand8Table {
+  uint16_t v[256];
+};
+
+ZX_HD constexpr Expand8Table make_expand8_table()
+{
+  Expand8Table t{};
+  for (uint32_t b = 0; b < 256; b++) t.v[b] = (uint16_t)expand8(b);
+  return t;
+}
+
+static constexpr Expand8Table kExpand8Host = make_expand8_table();
+
+#if ZX_DEVICE_BUILD
+// The workgroup's copy of the expand8 table; kernels that encode fill it
+// (expand8_fill) before any thread codes a block.
+static __shared__ uint16_t zx_expand8_lds[256];
+#endif
+
+ZX_HD uint32_t expand8_of(uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return zx_expand8_lds[b];
+#else
+  return kExpand8Host.v[b];
+#endif
+}
+
+// The one-bits of u (SIZE bits at most), the j-th from the bottom moved up
+// by j, as a 128-bit value (lo, hi).  Bytes are expanded through the table
+// and placed 8m + (one-bits below byte m) up; a byte range that is zero on
+// every lane of the wave is skipped.
+template <int SIZE>
+ZX_HD void expand_ones(uint64_t u, uint64_t &lo, uint64_t &hi)
+{
+  constexpr int BYTES = SIZE >= 32 ? 4 : (SIZE + 7) / 8;  // bytes of the low 32 bits
+  const uint32_t w0 = (uint32_t)u;
+  uint64_t e = expand8_of(w0 & 0xffu);
+  if constexpr (BYTES > 1) {
+    if (wave_any((w0 >> 8) != 0)) {
+      uint32_t r = popc32(w0 & 0xffu);
+      uint32_t b[BYTES];
+      uint64_t t[BYTES];
+      ZX_UNROLL
+      for (int m = 1; m < BYTES; m++) {
+        b[m] = (w0 >> (8 * m)) & 0xffu;
+        t[m] = expand8_of(b[m]);
+      }
+      ZX_UNROLL
+      for (int m = 1; m < BYTES; m++) {
+        e |= t[m] << (8 * m + r);  // entry below bit 15, shift at most 24 + 24
+        r += popc32(b[m]);
+      }
+    }
+  }
+  lo = e;
+  hi = 0;
+  if constexpr (SIZE > 32) {
+    const uint32_t w1 = (uint32_t)(u >> 32);
+    if (wave_any(w1 != 0)) {
+      uint32_t b[4];
+      uint64_t t[4];
+      ZX_UNROLL
+      for (int m = 0; m < 4; m++) {
+        b[m] = (w1 >> (8 * m)) & 0xffu;
+        t[m] = expand8_of(b[m]);
+      }
+      uint64_t f = 0;
+      uint32_t r = 0;
+      ZX_UNROLL
+      for (int m = 0; m < 4; m++) {
+        f |= t[m] << (8 * m + r);
+        r += popc32(b[m]);
+      }
+      const uint32_t sh = 32 + popc32(w0);  // where the upper half's expansion starts, 32..64
+      lo |= sh < 64 ? f << (sh & 63) : 0ull;
+      hi = f >> (64 - sh);
+    }
+  }
+}
+
 // Code of one bit plane x of a block whose first n coefficients are already
 // significant (bitplane.rs:560-605): the n low bits verbatim, then the group
 // tests over the rest.  Returns the code's length; the code is (lo, hi) and
@@ -649,23 +741,16 @@ ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, 
   const uint64_t rest = x >> n;
   clo = x & lowmask(n);
   if (rest == 0) return n + 1;  // one negative group test
-  // E holds one-bit j (input position q_j) at q_j + j; the group tests then
-  // read 1, E << 1 (the one-bits) and P << 2, P being E less its top bit
-  // (the positive tests between them)
-  uint64_t elo = 0, ehi = 0, tlo = 0, thi = 0, r = rest;
-  uint32_t j = 0, q = 0;
-  while (r) {
-    q = ctz64(r);
-    const uint32_t b = q + j;
-    const uint64_t t = 1ull << (b & 63);
-    tlo = b < 64 ? t : 0ull;
-    thi = b < 64 ? 0ull : t;
-    elo |= tlo;
-    ehi |= thi;
-    r &= r - 1;
-    j++;
-  }
-  const uint32_t tb = q + j - 1;  // position of E's top bit
+  // The tests find the one-bits of rest in order; the j-th, at q_j, puts its
+  // value bit at q_j + j + 1 of the tests' code and a positive test right
+  // after it.  E holds bit q_j + j for every j; the code is 1, E << 1 (the
+  // one-bits) and P << 2, P being E less its top bit (the positive tests).
+  uint64_t elo, ehi;
+  expand_ones<SIZE>(rest, elo, ehi);
+  const uint32_t q = 63u - clz64(rest);      // the last one-bit
+  const uint32_t tb = q + popc64(rest) - 1;  // E's top bit
+  const uint64_t t = 1ull << (tb & 63);
+  const uint64_t tlo = tb < 64 ? t : 0ull, thi = tb < 64 ? 0ull : t;
   const uint64_t plo = elo ^ tlo, phi = ehi ^ thi;
   // a one-bit at the last position is implied, with no test after it
   const bool implied = n + q == (uint32_t)SIZE - 1;

@@ -27,6 +27,14 @@ namespace zx {
 
 constexpr int kThreads = 256;
 
+// Fills the workgroup's expand8 table (zfpx_block.h); every kernel that
+// encodes calls it first, with all of its threads.
+__device__ __forceinline__ void expand8_fill()
+{
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) zx_expand8_lds[i] = (uint16_t)expand8(i);
+  __syncthreads();
+}
+
 // Rows of four contiguous values are loaded and stored as 16-byte vectors
 // when the host has checked that every row of every whole block is aligned.
 template <class S, int D>
@@ -97,6 +105,7 @@ __device__ __forceinline__ void store_block(const FieldDesc &f, uint64_t b, bool
 template <class S, int D, bool REV, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_encode(EncodeArgs a)
 {
+  expand8_fill();
   const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (b >= a.field.nblocks) return;
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
@@ -157,6 +166,7 @@ template <class S, int D, bool REV, bool OWN>
 __global__ __launch_bounds__(kThreads) void k_encode_fx(EncodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
+  expand8_fill();
   const uint32_t t = threadIdx.x;
   const FxGroup q = fx_group(blockIdx.x, p.T, a.field.nblocks, a.start, a.fixed);
   if constexpr (!OWN) {
@@ -215,6 +225,7 @@ __device__ __forceinline__ uint64_t batch_pos(const BatchArgs &a, const BatchJob
 template <class S, int D, bool REV, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_batch_encode(BatchArgs a)
 {
+  expand8_fill();
   const uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (g >= a.nblocks) return;
   const uint32_t j = job_of(a.firsts, a.njobs, g);
