@@ -527,23 +527,24 @@ struct CountSink {
 };
 
 // Word memories for sources: global words with a bound (past it reads zero,
-// bitstream/core.rs:151-157), or a staged copy addressed with a skew.
+// bitstream/core.rs:151-157), or a staged copy in LDS.
 struct GlobalWords {
-  static constexpr bool kReloadEachStep = false;
   const uint64_t *w;
   uint64_t n;
   ZX_HD uint64_t operator()(uint64_t i) const { return i < n ? w[i] : 0ull; }
 };
+// Word i at slot i + (i >> skew); the pad slot before each word whose index
+// is a multiple of 2^skew holds a second copy of that word, so the two slots
+// from word i's on hold words i and i + 1.
 struct StagedWords {
-  // LDS loads are cheap enough to refresh the second lookahead word on every
-  // step, which keeps the refill free of branches
-  static constexpr bool kReloadEachStep = true;
   const uint64_t *w;
   uint32_t skew;
-  ZX_HD uint64_t operator()(uint64_t i) const
+  // words i and i + 1 for the word i holding bit p
+  ZX_HD void pair_at(uint32_t p, uint64_t &lo, uint64_t &hi) const
   {
-    const uint32_t j = (uint32_t)i;
-    return w[j + (j >> skew)];
+    const uint64_t *q = w + ((p >> 6) + (p >> (skew + 6)));
+    lo = q[0];
+    hi = q[1];
   }
 };
 
@@ -586,9 +587,8 @@ struct Source {
       buf = (nxt >> 1) >> (63 - bits);
       idx++;
       nxt = nxt2;
-      if constexpr (!Mem::kReloadEachStep) nxt2 = mem(idx + 1);
+      nxt2 = mem(idx + 1);
     }
-    if constexpr (Mem::kReloadEachStep) nxt2 = mem(idx + 1);
   }
   ZX_HD uint64_t read(uint32_t n)
   {
@@ -615,8 +615,9 @@ struct Source<StagedWords> {
   }
   ZX_HD uint64_t peek() const
   {
-    const uint32_t i = p >> 6, s = p & 63;
-    const uint64_t lo = mem(i), hi = mem(i + 1);
+    const uint32_t s = p & 63;
+    uint64_t lo, hi;
+    mem.pair_at(p, lo, hi);
     return (lo >> s) | ((hi << 1) << (63 - s));
   }
   ZX_HD void consume(uint32_t n) { p += n; }
@@ -854,21 +855,26 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
         // group tests, one one-bit (or the final negative test) per
         // step, written without branches so the wave's lanes stay in step
         bool go = bits != 0 && n < (uint32_t)SIZE;
+        const bool tested = go;
+        bool one = false;
         while (go) {
           const uint64_t ahead = r.peek();
-          const bool one = (ahead & 1u) != 0;
-          bits -= 1;
-          const uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits);  // <= 63
+          one = (ahead & 1u) != 0;
+          // zeros before the next one-bit, up to the last coefficient (whose
+          // one-bit is implied) or the budget left after the test bit
+          const uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits - 1);  // <= 63
           const uint32_t zeros = ctz64(ahead >> 1);
           const bool hit = zeros < limit;
           const uint32_t adv = hit ? zeros : limit;
-          r.consume(one ? adv + 1 + (hit ? 1u : 0u) : 1u);
-          bits -= one ? adv + (hit ? 1u : 0u) : 0u;
-          x |= one ? 1ull << (n + adv) : 0ull;
+          const uint32_t step = one ? adv + 1 + (hit ? 1u : 0u) : 1u;
+          r.consume(step);
+          bits -= step;
+          x |= (ahead & 1u) << (n + adv);
           n += one ? adv + 1 : 0u;
-          m = one ? n : (uint32_t)SIZE;
           go = one && bits != 0 && n < (uint32_t)SIZE;
         }
+        // round_state's m: the count after a plane cut short, or all
+        if (tested) m = one ? n : (uint32_t)SIZE;
         pl[i] = (PW)x;
       }
     }

@@ -21,7 +21,9 @@
 //    atomic ORs (the host zeroes the stream's range beforehand); all other
 //    words are stored whole.
 // Decode stages the range (plus two words the reader looks ahead into) in the
-// contiguous layout; words at or past the stream's capacity read as zero.
+// contiguous layout, each pad slot holding a copy of the word after it so that
+// a word and its successor are always adjacent; words at or past the stream's
+// capacity read as zero.
 #pragma once
 
 #include "zfpx_field.h"
@@ -152,7 +154,11 @@ ZX_HD void fx_dec_stage(uint64_t *lds, const uint64_t *words, uint64_t nwords, c
                         const FxGroup &q, uint32_t i)
 {
   const uint64_t gw = q.w0 + i;
-  lds[fx_slot(i, p.skew)] = gw < nwords ? words[gw] : 0ull;
+  const uint64_t v = gw < nwords ? words[gw] : 0ull;
+  const uint32_t s = fx_slot(i, p.skew);
+  lds[s] = v;
+  // the pad slot before the word holds a copy of it (StagedWords)
+  if (i != 0 && (i & ((1u << p.skew) - 1)) == 0) lds[s - 1] = v;
 }
 
 // phase 2: thread t decodes its block into v; returns one past the highest
