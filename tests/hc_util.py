@@ -26,6 +26,8 @@ def lib():
                                   P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
                                   ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32]
         L.hc_compress.restype = ctypes.c_uint64
+        L.hc_divide.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]
+        L.hc_divide.restype = None
         L.hc_decompress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                     P(ctypes.c_int64), P(ctypes.c_int64), P(HcCfg),
                                     P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
@@ -71,3 +73,12 @@ def decompress(shape, dtype, ztype, oc, words, start_bit, fixed_bits, lens, rev,
                                  words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
                                  start_bit, fixed_bits, lp, lds_T)
     return out, loaded
+
+
+def divide(n, d):
+    """n // d (uint32 array) through the kernels' invariant-divisor division."""
+    import numpy as np
+    n = np.ascontiguousarray(n, dtype=np.uint32)
+    q = np.empty_like(n)
+    lib().hc_divide(n.ctypes.data, int(d), q.ctypes.data, n.size)
+    return q

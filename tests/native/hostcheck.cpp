@@ -18,13 +18,11 @@ static FieldDesc make_field(int dims, void *data, const int64_t n[3], const int6
 {
   FieldDesc f;
   f.data = data;
-  f.nblocks = 1;
   for (int a = 0; a < 3; a++) {
     f.n[a] = a < dims ? n[a] : 1;
     f.s[a] = a < dims ? s[a] : 0;
-    f.nb[a] = (uint32_t)((f.n[a] + 3) / 4);
-    f.nblocks *= f.nb[a];
   }
+  set_block_grid(f);
   return f;
 }
 
@@ -197,6 +195,13 @@ uint64_t hc_decompress(int type, int dims, int rev, void *data, const int64_t *n
   case 8: return decomp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
   default: return decomp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
   }
+}
+
+// n / d through the block grid's invariant-divisor path, for count values
+void hc_divide(const uint32_t *n, uint32_t d, uint32_t *q, uint64_t count)
+{
+  const Divisor v = make_divisor(d);
+  for (uint64_t i = 0; i < count; i++) q[i] = divide(n[i], v);
 }
 
 }  // extern "C"

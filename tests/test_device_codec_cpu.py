@@ -133,3 +133,20 @@ def test_roundings_match_oracle(dims, ztype):
             run_case(arr, ztype, cfg, 0)
         except AssertionError as e:
             raise AssertionError(f"{name} rounding={rnd} tight={tight}: {e}") from None
+
+
+def test_block_grid_division_is_exact():
+    """The block-grid coordinates use a multiply-and-shift division by the
+    grid's extents; it must equal integer division for every dividend."""
+    rng = np.random.default_rng(21)
+    edge = np.array([0, 1, 2, 3, 0x7fffffff, 0x80000000, 0x80000001, 0xfffffffe, 0xffffffff],
+                    dtype=np.uint64)
+    divisors = [1, 2, 3, 5, 7, 64, 127, 128, 129, 1000, 4097, 65535, 65536, 65537,
+                (1 << 31) - 1, 1 << 31, (1 << 31) + 1, 0xfffffffe, 0xffffffff]
+    divisors += [int(d) for d in rng.integers(1, 1 << 32, size=40, dtype=np.uint64)]
+    for d in divisors:
+        n = np.concatenate([edge, rng.integers(0, 1 << 32, size=4000, dtype=np.uint64),
+                            np.array([k * d + r for k in (1, 2, 1000) for r in (0, d - 1)
+                                      if k * d + r < (1 << 32)], dtype=np.uint64)])
+        got = H.divide(n.astype(np.uint32), d)
+        assert np.array_equal(got.astype(np.uint64), n // np.uint64(d)), d

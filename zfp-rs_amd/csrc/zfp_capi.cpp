@@ -726,9 +726,8 @@ zx::FieldDesc field_desc(const Plan &p, void *origin)
   for (int a = 0; a < 3; a++) {
     f.n[a] = p.n[a];
     f.s[a] = p.s[a];
-    f.nb[a] = p.nb[a];
   }
-  f.nblocks = p.nblocks;
+  zx::set_block_grid(f);
   return f;
 }
 

@@ -260,6 +260,28 @@ ZX_HD bool wave_all(bool p) { return !wave_any(!p); }
 
 // ---------------------------------------------------------------- bit planes
 
+// Byte select of v_perm_b32: byte k of the result is byte sel_k of the
+// eight bytes hi:lo (0-3 from lo, 4-7 from hi); selector values 0-7 only.
+ZX_HD uint32_t perm_bytes(uint32_t hi, uint32_t lo, uint32_t sel)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+  const uint64_t v = ((uint64_t)hi << 32) | lo;
+  uint32_t r = 0;
+  for (int k = 0; k < 4; k++) r |= (uint32_t)((v >> (8 * ((sel >> (8 * k)) & 7))) & 0xffu) << (8 * k);
+  return r;
+#endif
+}
+
+// Selector that keeps bytes 0-3 of lo except byte k, which takes byte b of hi:lo.
+ZX_HD constexpr uint32_t byte_insert_sel(int k, int b)
+{
+  uint32_t s = 0;
+  for (int i = 0; i < 4; i++) s |= (uint32_t)(i == k ? b : i) << (8 * i);
+  return s;
+}
+
 // Stages of the 32x32 bit-matrix transpose (row r bit c <-> row c bit r).
 // The stage of width j swaps index bit log2(j) of rows and columns; the five
 // stages commute, so any subset of rows can be finished independently.
@@ -364,7 +386,20 @@ ZX_HD void chunk_to_coeffs(const typename PWord<SIZE>::T (&pl)[CH], int c,
       ZX_UNROLL
       for (int f = 0; f < 32 / CH; f++) {
         const int j = 32 * g + CH * f + i;
-        if (j < SIZE) u[j < SIZE ? j : 0] |= (U)((t[i] >> (CH * f)) & FM) << (CH * c);
+        if (j >= SIZE) continue;
+        U &d = u[j < SIZE ? j : 0];
+        if constexpr (CH == 8) {
+          // byte f of t[i] into byte c of the coefficient (empty until now)
+          const int h = (8 * c) / 32, k = c % 4;
+          const uint32_t w = (uint32_t)((uint64_t)d >> (32 * h));
+          const uint32_t r = perm_bytes(t[i], w, byte_insert_sel(k, 4 + f));
+          if constexpr (sizeof(U) == 8)
+            d = h ? (U)(((uint64_t)r << 32) | (uint32_t)(uint64_t)d) : (U)((((uint64_t)d >> 32) << 32) | r);
+          else
+            d = (U)r;
+        } else {
+          d |= (U)((t[i] >> (CH * f)) & FM) << (CH * c);
+        }
       }
     }
   }
@@ -658,8 +693,9 @@ ZX_HD constexpr Expand8Table make_expand8_table()
 static constexpr Expand8Table kExpand8Host = make_expand8_table();
 
 #if ZX_DEVICE_BUILD
-// The workgroup's copy of the expand8 table; kernels that encode fill it
-// (expand8_fill) before any thread codes a block.
+// The table in device memory, and the workgroup's copy in LDS, which kernels
+// that encode fill (expand8_fill) before any thread codes a block.
+static __constant__ Expand8Table kExpand8Dev = make_expand8_table();
 static __shared__ uint16_t zx_expand8_lds[256];
 #endif
 

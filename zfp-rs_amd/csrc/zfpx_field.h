@@ -13,13 +13,61 @@
 
 namespace zx {
 
+// Division by an invariant 32-bit divisor with one high multiply and shifts
+// (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, fig. 4.1): exact for every 32-bit dividend and
+// every divisor 1 <= d < 2^32.
+struct Divisor {
+  uint32_t d, m, s1, s2;
+};
+
+ZX_HD Divisor make_divisor(uint32_t d)
+{
+  uint32_t l = 0;  // ceil(log2 d)
+  while (l < 32 && ((uint64_t)1 << l) < d) l++;
+  Divisor v;
+  v.d = d;
+  v.m = (uint32_t)(((((uint64_t)1 << l) - d) << 32) / d + 1);
+  v.s1 = l < 1 ? l : 1u;
+  v.s2 = l > 0 ? l - 1 : 0u;
+  return v;
+}
+
+ZX_HD uint32_t mulhi32(uint32_t a, uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umulhi(a, b);
+#else
+  return (uint32_t)(((uint64_t)a * b) >> 32);
+#endif
+}
+
+ZX_HD uint32_t divide(uint32_t n, const Divisor &v)
+{
+  const uint32_t t = mulhi32(v.m, n);
+  return (t + ((n - t) >> v.s1)) >> v.s2;
+}
+
 struct FieldDesc {
   void *data;          // element [0,0,0]; negative strides reach below it
   int64_t n[3];        // extents (1 for unused axes)
   int64_t s[3];        // element strides
   uint32_t nb[3];      // blocks per axis
   uint64_t nblocks;
+  Divisor bx, by;      // nb[0], nb[1] as divisors (set_block_grid)
 };
+
+// Sets the block grid of a field whose extents are set.
+ZX_HD void set_block_grid(FieldDesc &f)
+{
+  f.nblocks = 1;
+  for (int a = 0; a < 3; a++) {
+    f.nb[a] = (uint32_t)((f.n[a] + 3) / 4);
+    f.nblocks *= f.nb[a];
+  }
+  f.bx = make_divisor(f.nb[0]);
+  f.by = make_divisor(f.nb[1]);
+}
 
 ZX_HD uint32_t pad_src(uint32_t i, uint32_t n) { return i < n ? i : (i == 3 ? 0u : n - 1); }
 
@@ -38,13 +86,13 @@ ZX_HD BlockPos block_pos(const FieldDesc &f, uint64_t b)
   if constexpr (D == 1) {
     c[0] = bb;
   } else if constexpr (D == 2) {
-    c[0] = bb % f.nb[0];
-    c[1] = bb / f.nb[0];
+    c[1] = divide(bb, f.bx);
+    c[0] = bb - c[1] * f.nb[0];
   } else {
-    c[0] = bb % f.nb[0];
-    const uint32_t t = bb / f.nb[0];
-    c[1] = t % f.nb[1];
-    c[2] = t / f.nb[1];
+    const uint32_t t = divide(bb, f.bx);
+    c[0] = bb - t * f.nb[0];
+    c[2] = divide(t, f.by);
+    c[1] = t - c[2] * f.nb[1];
   }
   p.origin = 0;
   p.full = true;

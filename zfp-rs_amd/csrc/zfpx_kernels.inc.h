@@ -31,7 +31,7 @@ constexpr int kThreads = 256;
 // encodes calls it first, with all of its threads.
 __device__ __forceinline__ void expand8_fill()
 {
-  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) zx_expand8_lds[i] = (uint16_t)expand8(i);
+  for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) zx_expand8_lds[i] = kExpand8Dev.v[i];
   __syncthreads();
 }
 
@@ -57,6 +57,10 @@ __device__ __forceinline__ void gather_rows(const FieldDesc &f, const BlockPos &
       v[4 * r + 0] = a.x; v[4 * r + 1] = a.y; v[4 * r + 2] = b.x; v[4 * r + 3] = b.y;
     }
   }
+  // An empty barrier: without it the compiler sinks these loads into a block
+  // shared with the element loads of partial blocks, splitting each 16-byte
+  // load into four with an address apiece.
+  asm volatile("" ::: "memory");
 }
 
 template <class S, int D>
