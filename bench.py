@@ -27,6 +27,7 @@ Also reported:
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rate 8] [--n 512]
 """
 import argparse
+import ctypes
 import json
 import os
 import pathlib
@@ -429,16 +430,25 @@ def main():
     s.set_timing(True)
 
     enc_ms, dec_ms = [], []
+    # the steps call the C ABI directly, as a C caller would: fields described
+    # once, then zfp_stream_rewind + zfp_compress / zfp_decompress per step
+    L = Z.lib()
+    zs = s.zs
+    fin, fout = Z.make_field(x), Z.make_field(out)
+    pin, pout = ctypes.byref(fin), ctypes.byref(fout)
+    call_ms = ctypes.c_double(0)
 
     def step(record):
-        s.rewind()
-        c = s.compress(x)
+        L.zfp_stream_rewind(zs)
+        c = L.zfp_compress(zs, pin)
         if record:
-            enc_ms.append(s.kernel_ms()[0])
-        s.rewind()
-        d = s.decompress(out)
+            enc_ms.append(L.zfp_stream_kernel_ms(zs, ctypes.byref(call_ms)))
+        L.zfp_stream_rewind(zs)
+        d = L.zfp_decompress(zs, pout)
         if record:
-            dec_ms.append(s.kernel_ms()[0])
+            dec_ms.append(L.zfp_stream_kernel_ms(zs, ctypes.byref(call_ms)))
+        if not (c and d):
+            raise RuntimeError("zfp_compress / zfp_decompress failed")
         return c, d
 
     print(f"[main] field ready, {args.warmup} warmup + {args.steps} steps", file=sys.stderr, flush=True)

@@ -898,9 +898,15 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   // device words from w0 on
   const uint64_t avail = s->nwords > w0 ? s->nwords - w0 : 0;
   uint64_t need = avail;
+  // A fixed-rate decode loads no word past the one holding its last bit, so
+  // when the stream holds that word nothing can be overread and the highest
+  // loaded word is not needed.
+  bool whole = false;
   if (fixed) {
     const uint64_t end_rel = a.start + p.nblocks * (uint64_t)c.maxbits;
     need = std::min<uint64_t>(avail, (end_rel + 63) / 64);
+    whole = avail >= (end_rel + 63) / 64;
+    if (whole) a.maxword = nullptr;
   }
   const uint64_t *dwords;
   if (s->device) {
@@ -936,11 +942,11 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
     a.offsets = offs;
   }
   if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
-  ZCHECK(zx_launch_fill(maxword, 1, 0, st));
+  if (!whole) ZCHECK(zx_launch_fill(maxword, 1, 0, st));
   tm.mark(1);
   ZCHECK(launch_decode(p.type, p.dims, rev, a, nullptr, st));
   tm.mark(2);
-  ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 1, maxword, 8, hipMemcpyDeviceToHost, st));
+  if (!whole) ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 1, maxword, 8, hipMemcpyDeviceToHost, st));
   if (staged) {
     const size_t span = (size_t)(p.imax - p.imin + 1) * p.esz;
     char *lo = (char *)f->data + p.imin * (int64_t)p.esz;
@@ -955,7 +961,8 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   // park + align: the cursor at the word boundary after the last block.  A
   // load past the end (not a skip over missing padding) is a truncation.
   const uint64_t w1 = (end + 63) / 64;
-  const uint64_t loaded_end = w0 + ((uint64_t *)W.one)[1];  // one past the highest loaded word
+  // one past the highest loaded word
+  const uint64_t loaded_end = whole ? w1 : w0 + ((uint64_t *)W.one)[1];
   const bool over = loaded_end > std::max<uint64_t>(s->nwords, seek_word0);
   s->word_pos = (size_t)w1;
   s->bits = 0;

@@ -29,7 +29,7 @@ struct DecodeArgs {
   uint64_t start;
   uint32_t fixed;
   const uint64_t *offsets;
-  uint64_t *maxword;        // one past the highest word loaded (written by the last block)
+  uint64_t *maxword;        // one past the highest word loaded (written by the last block), or null
   bool vec;
 };
 

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
   decode_values<S, D, REV>(r, v, a.cfg);
   // blocks are contiguous, so the last one loads the highest word
-  if (b == a.field.nblocks - 1) *a.maxword = loaded_words(r, pos, a.cfg.minbits);
+  if (a.maxword && b == a.field.nblocks - 1) *a.maxword = loaded_words(r, pos, a.cfg.minbits);
   store_block<S, D>(a.field, b, a.vec, v);
 }
 
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
   const uint64_t loaded = fx_dec_block<S, D, REV>(lds, p, q, t, a.fixed, v, a.cfg);
   // blocks are contiguous, so the last one loads the highest word
-  if (q.b0 + t == a.field.nblocks - 1) *a.maxword = loaded;
+  if (a.maxword && q.b0 + t == a.field.nblocks - 1) *a.maxword = loaded;
   store_block<S, D>(a.field, q.b0 + t, a.vec, v);
 }
 
