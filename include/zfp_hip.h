@@ -66,6 +66,34 @@ size_t zfp_hip_compress_batch(zfp_stream *const *streams, const zfp_field *const
 size_t zfp_hip_decompress_batch(zfp_stream *const *streams, zfp_field *const *fields, size_t count,
                                 size_t *sizes, uint flags);
 
+/* Planned batches: a batch of the above planned once and replayed from a
+ * captured HIP graph, for callers that code the same arrays again and again
+ * (a checkpoint of the same fields every step, a fixed set of serving
+ * buffers).  No counterpart in the reference; an extension beside
+ * zfp_hip_compress_batch.
+ * zfp_hip_batch_plan: plans the arrays (encode != 0: compress, else
+ * decompress) from the streams' current cursors, tunes the launches' tiling
+ * (blocks per workgroup, and the LDS slice of large fixed-rate arrays) by
+ * timing the whole batch a few times -- those runs write what a run writes --
+ * unless ZFP_HIP_BATCH_NO_TUNE is set, and captures the device work into a
+ * graph.  Returns NULL unless every array is one the batched launches take
+ * (device data and stream words, one array per stream, at least one block).
+ * zfp_hip_batch_run: rewinds each stream to its planned cursor, replays the
+ * graph on the first stream's HIP stream, and leaves cursors and sizes as the
+ * batch call would.  Returns the arrays coded; 0 (all sizes 0) when a stream,
+ * its words, configuration or offset index is not the one planned.  The
+ * fields' data pointers, shapes and strides are fixed at plan time.
+ * zfp_hip_batch_tiling: the chosen blocks per workgroup and slice (0: the
+ * default) and the tuned run's time in ms (-1 when not tuned). */
+#define ZFP_HIP_BATCH_NO_TUNE 2u
+typedef struct zfp_hip_batch zfp_hip_batch;
+zfp_hip_batch *zfp_hip_batch_plan(int encode, zfp_stream *const *streams, zfp_field *const *fields,
+                                  size_t count, uint flags);
+size_t zfp_hip_batch_run(zfp_hip_batch *batch, size_t *sizes);
+size_t zfp_hip_batch_count(const zfp_hip_batch *batch);
+double zfp_hip_batch_tiling(const zfp_hip_batch *batch, uint *threads, uint *slice);
+void zfp_hip_batch_free(zfp_hip_batch *batch);
+
 /* Device the library runs on (the current HIP device when first used). */
 int zfp_hip_device(void);
 

@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <unordered_set>
 #include <vector>
@@ -24,6 +25,7 @@
 #include "../../include/zfp.h"
 #include "../../include/zfp_hip.h"
 #include "zfpx_api.h"
+#include "zfpx_fx.h"
 
 extern "C" {
 const size_t stream_word_bits = 64;
@@ -787,7 +789,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   Timer tm(xp, st);
   if (!stage_field(W, f, p, true, st, dorigin, staged)) return false;
   tm.mark(0);
-  zx::EncodeArgs a;
+  zx::EncodeArgs a{};
   a.field = field_desc(p, dorigin);
   a.cfg = c;
   a.vec = vec_ok(p, dorigin);
@@ -884,7 +886,7 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   Timer tm(xp, st);
   if (!stage_field(W, f, p, !contiguous(p), st, dorigin, staged)) return false;
   tm.mark(0);
-  zx::DecodeArgs a;
+  zx::DecodeArgs a{};
   a.field = field_desc(p, dorigin);
   a.cfg = c;
   a.vec = vec_ok(p, dorigin);
@@ -1037,7 +1039,7 @@ bool grow_pinned(void *&p, size_t &cap, size_t need)
 }
 
 // Checks an array as zfp_compress / zfp_decompress do; false when the call
-// would return 0 before coding.
+// would fail before touching the stream.
 bool batch_plan(zfp_stream *z, const zfp_field *f, bool decode, Plan &p)
 {
   if (!z || !f || !z->stream || !f->data) return false;
@@ -1047,15 +1049,34 @@ bool batch_plan(zfp_stream *z, const zfp_field *f, bool decode, Plan &p)
   return true;
 }
 
-size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_t count, size_t *sizes,
-                 uint flags)
-{
-  if (!zs || !fs || !sizes) return 0;
-  size_t ok = 0;
+// A batch as planned on the host: the arrays the batched launches take,
+// ordered by class, with their chunk and table layout.
+struct BatchSet {
+  bool encode = true;
   std::vector<BatchItem> items;
-  std::vector<size_t> later;  // arrays for the single-call path, in input order
+  std::vector<size_t> later;  // arrays for the single-array path, in input order
+  std::vector<uint64_t> firsts, fill_first, blk_first;
+  uint64_t var_total = 0, fill_chunks = 0, blk_chunks = 0;
+  size_t jbytes = 0, fbytes = 0, rbytes = 0;
+};
+
+// Device side of a batch: pinned tables and results, their device copy, the
+// variable-rate scratch, and the tiling of the launches.
+struct BatchDev {
+  void *hbuf = nullptr, *dbuf = nullptr;
+  void *lens = nullptr, *offs = nullptr, *tiles = nullptr;
+  uint32_t threads = 256;  // blocks per workgroup of the class launches
+  uint32_t slice = 0;      // blocks per workgroup of arrays launched alone (0: default)
+};
+
+// Plans the arrays; sizes[] is zeroed.  Arrays the batch does not take
+// (invalid, host-resident, a stream met before, no blocks) go to `later`.
+void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_t count,
+                   size_t *sizes, uint flags, BatchSet &S)
+{
+  S.encode = encode;
   std::unordered_set<const bitstream *> seen;
-  items.reserve(count);
+  S.items.reserve(count);
   for (size_t i = 0; i < count; i++) {
     sizes[i] = 0;
     zfp_stream *z = zs[i];
@@ -1064,16 +1085,16 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
     if (!batch_plan(z, f, !encode, p)) continue;
     bitstream *s = z->stream;
     if (!seen.insert(s).second) {
-      later.push_back(i);
+      S.later.push_back(i);
       continue;
     }
     if (p.nblocks == 0) {  // nothing to code: the calls' own handling
-      later.push_back(i);
+      S.later.push_back(i);
       continue;
     }
     const bool dev = s->device && ((flags & ZFP_HIP_BATCH_DEVICE_FIELDS) || is_device_ptr(f->data));
     if (!dev) {
-      later.push_back(i);
+      S.later.push_back(i);
       continue;
     }
     BatchItem it;
@@ -1121,203 +1142,358 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
     }
     const int variant = ((int)p.type - (int)zfp_type_int32) * 6 + (p.dims - 1) * 2 + (rev ? 1 : 0);
     it.cls = fixed && p.nblocks >= kSoloBlocks ? kSoloClass + variant : variant * 2 + (fixed ? 0 : 1);
-    items.push_back(it);
+    S.items.push_back(it);
   }
-
-  if (!items.empty()) {
-    std::stable_sort(items.begin(), items.end(),
-                     [](const BatchItem &a, const BatchItem &b) { return a.cls < b.cls; });
-    const size_t n = items.size();
-    std::vector<uint64_t> firsts(n);
-    uint64_t var_total = 0;
-    for (size_t k = 0; k < n; k++) {
-      if (!items[k].job.fixed) {
-        items[k].job.lbase = var_total;
-        var_total += items[k].plan.nblocks;
-      }
+  if (S.items.empty()) return;
+  std::stable_sort(S.items.begin(), S.items.end(),
+                   [](const BatchItem &a, const BatchItem &b) { return a.cls < b.cls; });
+  const size_t n = S.items.size();
+  S.firsts.assign(n, 0);
+  S.fill_first.assign(n, 0);
+  S.blk_first.assign(n, 0);
+  for (size_t k = 0; k < n; k++) {
+    if (!S.items[k].job.fixed) {
+      S.items[k].job.lbase = S.var_total;
+      S.var_total += S.items[k].plan.nblocks;
     }
+  }
+  for (size_t k = 0; k < n;) {
+    size_t e = k;
+    uint64_t acc = 0;
+    while (e < n && S.items[e].cls == S.items[k].cls) {
+      S.firsts[e] = acc;
+      acc += S.items[e].plan.nblocks;
+      e++;
+    }
+    k = e;
+  }
+  // chunks of the per-array helpers: word fills and index copies
+  for (size_t k = 0; k < n; k++) {
+    const zx::BatchJob &J = S.items[k].job;
+    S.fill_first[k] = S.fill_chunks;
+    S.blk_first[k] = S.blk_chunks;
+    if (encode && J.fill) {
+      uint64_t words = J.nwords;
+      if (J.fixed) words = std::min<uint64_t>(words, (J.start + S.items[k].plan.nblocks * J.fixed + 63) / 64);
+      S.fill_chunks += zx_batch_chunks(words);
+    }
+    if (!J.fixed) S.blk_chunks += zx_batch_chunks(S.items[k].plan.nblocks);
+  }
+  S.jbytes = (n * sizeof(zx::BatchJob) + 15) / 16 * 16;
+  S.fbytes = n * 8 * 3;  // firsts, fill_first, blk_first
+  S.rbytes = n * 16;
+}
+
+// Bytes of the pinned and device table buffers, and of the scratch.
+size_t batch_host_bytes(const BatchSet &S) { return std::max(S.jbytes + S.fbytes, S.rbytes); }
+size_t batch_dev_bytes(const BatchSet &S) { return S.jbytes + S.fbytes + S.rbytes; }
+
+// Writes the job table and chunk tables into D.hbuf (pinned); each job's
+// results go to its slot after the tables in D.dbuf.
+void batch_tables(BatchSet &S, const BatchDev &D)
+{
+  const size_t n = S.items.size();
+  uint64_t *dres = (uint64_t *)((char *)D.dbuf + S.jbytes + S.fbytes);
+  zx::BatchJob *hjobs = (zx::BatchJob *)D.hbuf;
+  for (size_t k = 0; k < n; k++) {
+    S.items[k].job.result = dres + 2 * k;
+    hjobs[k] = S.items[k].job;
+  }
+  memcpy((char *)D.hbuf + S.jbytes, S.firsts.data(), n * 8);
+  memcpy((char *)D.hbuf + S.jbytes + n * 8, S.fill_first.data(), n * 8);
+  memcpy((char *)D.hbuf + S.jbytes + n * 16, S.blk_first.data(), n * 8);
+}
+
+// Records the batch's device work on `st`: the table upload (when `upload`),
+// length passes and scan of variable-rate arrays, end bits, word fills, the
+// coding launches, index copies, and the results' download into D.hbuf.  No
+// host synchronisation and no memsets, so the sequence can be captured.
+bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_t st)
+{
+  const bool encode = S.encode;
+  const size_t n = S.items.size();
+  const zx::BatchJob *djobs = (const zx::BatchJob *)D.dbuf;
+  const uint64_t *dfirsts = (const uint64_t *)((char *)D.dbuf + S.jbytes);
+  const uint64_t *dfill_first = dfirsts + n;
+  const uint64_t *dblk_first = dfirsts + 2 * n;
+  uint64_t *dres = (uint64_t *)((char *)D.dbuf + S.jbytes + S.fbytes);
+  uint32_t *lens = (uint32_t *)D.lens;
+  uint64_t *offs = (uint64_t *)D.offs;
+  const auto &items = S.items;
+  auto run = [&](bool var_only, bool count_pass) -> bool {
     for (size_t k = 0; k < n;) {
       size_t e = k;
-      uint64_t acc = 0;
-      while (e < n && items[e].cls == items[k].cls) {
-        firsts[e] = acc;
-        acc += items[e].plan.nblocks;
-        e++;
-      }
-      k = e;
-    }
-    zfp_stream *z0 = zs[items[0].input];
-    const hipStream_t st = hip_stream_of(z0);
-    Workspace &W = ws();
-    BatchBuffers &B = batch_buffers();
-    std::lock_guard<std::mutex> lock(W.mu);
-    // chunks of the per-array helpers: word fills and index copies
-    std::vector<uint64_t> fill_first(n), blk_first(n);
-    uint64_t fill_chunks = 0, blk_chunks = 0;
-    for (size_t k = 0; k < n; k++) {
-      const zx::BatchJob &J = items[k].job;
-      fill_first[k] = fill_chunks;
-      blk_first[k] = blk_chunks;
-      if (encode && J.fill) {
-        uint64_t words = J.nwords;
-        if (J.fixed) words = std::min<uint64_t>(words, (J.start + items[k].plan.nblocks * J.fixed + 63) / 64);
-        fill_chunks += zx_batch_chunks(words);
-      }
-      if (!J.fixed) blk_chunks += zx_batch_chunks(items[k].plan.nblocks);
-    }
-    const size_t jbytes = (n * sizeof(zx::BatchJob) + 15) / 16 * 16;
-    const size_t fbytes = n * 8 * 3;  // firsts, fill_first, blk_first
-    const size_t rbytes = n * 16;
-    bool good = grow_pinned(B.hbuf, B.hcap, std::max(jbytes + fbytes, rbytes)) &&
-                Workspace::grow(B.dbuf, B.dcap, jbytes + fbytes + rbytes);
-    if (good && var_total) {
-      good = Workspace::grow(W.lens, W.lens_cap, var_total * 4) &&
-             Workspace::grow(W.offs, W.offs_cap, (var_total + 1) * 8) &&
-             Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(var_total) * 8);
-    }
-    if (good) {
-      uint64_t *dres = (uint64_t *)((char *)B.dbuf + jbytes + fbytes);
-      zx::BatchJob *hjobs = (zx::BatchJob *)B.hbuf;
-      for (size_t k = 0; k < n; k++) {
-        items[k].job.result = dres + 2 * k;
-        hjobs[k] = items[k].job;
-      }
-      memcpy((char *)B.hbuf + jbytes, firsts.data(), n * 8);
-      memcpy((char *)B.hbuf + jbytes + n * 8, fill_first.data(), n * 8);
-      memcpy((char *)B.hbuf + jbytes + n * 16, blk_first.data(), n * 8);
-      const zx::BatchJob *djobs = (const zx::BatchJob *)B.dbuf;
-      const uint64_t *dfirsts = (const uint64_t *)((char *)B.dbuf + jbytes);
-      const uint64_t *dfill_first = dfirsts + n;
-      const uint64_t *dblk_first = dfirsts + 2 * n;
-      uint32_t *lens = (uint32_t *)W.lens;
-      uint64_t *offs = (uint64_t *)W.offs;
-      auto run = [&](bool var_only, bool count) -> bool {
-        for (size_t k = 0; k < n;) {
-          size_t e = k;
-          uint64_t nb = 0;
-          while (e < n && items[e].cls == items[k].cls) nb += items[e++].plan.nblocks;
-          const int cls = items[k].cls;
-          if (cls >= kSoloClass) {
-            if (!var_only) {
-              // arrays launched on their own, through the single-array kernels
-              for (size_t q = k; q < e; q++) {
-                const zx::BatchJob &J = items[q].job;
-                const int v = cls - kSoloClass;
-                const zfp_type t = (zfp_type)(zfp_type_int32 + v / 6);
-                const int dims = (v % 6) / 2 + 1;
-                const bool rev = v & 1;
-                hipError_t r;
-                if (encode) {
-                  zx::EncodeArgs ea;
-                  ea.field = J.field;
-                  ea.cfg = J.cfg;
-                  ea.words = J.words;
-                  ea.nwords = J.nwords;
-                  ea.start = J.start;
-                  ea.fixed = J.fixed;
-                  ea.offsets = nullptr;
-                  ea.lengths = nullptr;
-                  ea.vec = J.vec;
-                  r = launch_encode(t, dims, rev, ea, st);
-                } else {
-                  zx::DecodeArgs da;
-                  da.field = J.field;
-                  da.cfg = J.cfg;
-                  da.words = J.words;
-                  da.nwords = J.nwords;
-                  da.start = J.start;
-                  da.fixed = J.fixed;
-                  da.offsets = nullptr;
-                  da.maxword = dres + 2 * q + 1;
-                  da.vec = J.vec;
-                  r = launch_decode(t, dims, rev, da, nullptr, st);
-                }
-                if (r != hipSuccess) {
-                  (void)hipGetLastError();
-                  return false;
-                }
-              }
+      uint64_t nb = 0;
+      while (e < n && items[e].cls == items[k].cls) nb += items[e++].plan.nblocks;
+      const int cls = items[k].cls;
+      if (cls >= kSoloClass) {
+        if (!var_only) {
+          // arrays launched on their own, through the single-array kernels
+          for (size_t q = k; q < e; q++) {
+            const zx::BatchJob &J = items[q].job;
+            const int v = cls - kSoloClass;
+            const zfp_type t = (zfp_type)(zfp_type_int32 + v / 6);
+            const int dims = (v % 6) / 2 + 1;
+            const bool rev = v & 1;
+            hipError_t r;
+            if (encode) {
+              zx::EncodeArgs ea{};
+              ea.field = J.field;
+              ea.cfg = J.cfg;
+              ea.words = J.words;
+              ea.nwords = J.nwords;
+              ea.start = J.start;
+              ea.fixed = J.fixed;
+              ea.vec = J.vec;
+              ea.slice = D.slice;
+              r = launch_encode(t, dims, rev, ea, st);
+            } else {
+              zx::DecodeArgs da{};
+              da.field = J.field;
+              da.cfg = J.cfg;
+              da.words = J.words;
+              da.nwords = J.nwords;
+              da.start = J.start;
+              da.fixed = J.fixed;
+              da.maxword = dres + 2 * q + 1;
+              da.vec = J.vec;
+              da.slice = D.slice;
+              r = launch_decode(t, dims, rev, da, nullptr, st);
             }
-          } else if (!var_only || (cls & 1)) {
-            zx::BatchArgs a;
-            a.jobs = djobs + k;
-            a.firsts = dfirsts + k;
-            a.njobs = (uint32_t)(e - k);
-            a.nblocks = nb;
-            a.lengths = lens;
-            a.offsets = offs;
-            const int v = cls >> 1;
-            const hipError_t r = encode ? kBEnc[v / 6][v % 6](a, count, st) : kBDec[v / 6][v % 6](a, count, st);
             if (r != hipSuccess) {
               (void)hipGetLastError();
               return false;
             }
           }
-          k = e;
         }
-        return true;
-      };
-      good = hipMemcpyAsync(B.dbuf, B.hbuf, jbytes + fbytes, hipMemcpyHostToDevice, st) == hipSuccess;
-      if (good && var_total) {
-        if (encode) good = run(true, true);
-        else
-          good = zx_batch_gather_index(djobs, dblk_first, (uint32_t)n, blk_chunks, lens, st) == hipSuccess &&
-                 run(true, true);
-        good = good && zx_launch_scan(lens, var_total, 0, offs, (uint64_t *)W.tiles, st) == hipSuccess;
+      } else if (!var_only || (cls & 1)) {
+        zx::BatchArgs a{};
+        a.jobs = djobs + k;
+        a.firsts = dfirsts + k;
+        a.njobs = (uint32_t)(e - k);
+        a.nblocks = nb;
+        a.lengths = lens;
+        a.offsets = offs;
+        a.threads = D.threads;
+        const int v = cls >> 1;
+        const hipError_t r = encode ? kBEnc[v / 6][v % 6](a, count_pass, st) : kBDec[v / 6][v % 6](a, count_pass, st);
+        if (r != hipSuccess) {
+          (void)hipGetLastError();
+          return false;
+        }
       }
-      good = good && zx_batch_ends(djobs, (uint32_t)n, offs, st) == hipSuccess;
-      if (good && encode) good = zx_batch_fill(djobs, dfill_first, (uint32_t)n, fill_chunks, st) == hipSuccess;
-      good = good && run(false, false);
-      if (good && encode && var_total)
-        good = zx_batch_keep_index(djobs, dblk_first, (uint32_t)n, blk_chunks, lens, st) == hipSuccess;
-      good = good && hipMemcpyAsync(B.hbuf, dres, rbytes, hipMemcpyDeviceToHost, st) == hipSuccess;
-      good = good && hipStreamSynchronize(st) == hipSuccess;
-      if (!good) (void)hipGetLastError();
+      k = e;
     }
-    if (good) {
-      const uint64_t *res = (const uint64_t *)B.hbuf;
-      for (size_t k = 0; k < n; k++) {
-        const BatchItem &it = items[k];
-        zfp_stream *z = zs[it.input];
-        bitstream *s = z->stream;
-        const uint64_t end = 64 * it.w0 + res[2 * k];
-        const uint64_t w1 = (end + 63) / 64;
-        if (encode) {
-          // the cursor after the last block, then flush (core.rs:536-542)
-          if (w1 > s->nwords) s->overflowed = true;
-          s->word_pos = (size_t)w1;
-          s->bits = 0;
-          s->buffer = 0;
-          if (!it.job.fixed && s->index) {
-            s->index_n = it.plan.nblocks;
-            s->index_start = it.start;
-            s->index_key = cfg_key(cfg_of(z), it.plan);
-          }
-          if (!s->overflowed) {
-            sizes[it.input] = s->word_pos * 8;
-            ok++;
-          }
-        } else {
-          const uint64_t loaded_end = it.w0 + res[2 * k + 1];
-          s->word_pos = (size_t)w1;
-          s->bits = 0;
-          s->buffer = 0;
-          s->seek_word = s->word_pos;
-          s->overread = loaded_end > std::max<uint64_t>(s->nwords, it.seek_word0);
-          if (!s->overread_now()) {
-            sizes[it.input] = s->word_pos * 8;
-            ok++;
-          }
-        }
+    return true;
+  };
+  bool good = true;
+  if (upload) good = hipMemcpyAsync(D.dbuf, D.hbuf, S.jbytes + S.fbytes, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (good && S.var_total) {
+    if (encode) good = run(true, true);
+    else
+      good = zx_batch_gather_index(djobs, dblk_first, (uint32_t)n, S.blk_chunks, lens, st) == hipSuccess &&
+             run(true, true);
+    good = good && zx_launch_scan(lens, S.var_total, 0, offs, (uint64_t *)D.tiles, st) == hipSuccess;
+  }
+  good = good && zx_batch_ends(djobs, (uint32_t)n, offs, st) == hipSuccess;
+  if (good && encode) good = zx_batch_fill(djobs, dfill_first, (uint32_t)n, S.fill_chunks, st) == hipSuccess;
+  good = good && run(false, false);
+  if (good && encode && S.var_total)
+    good = zx_batch_keep_index(djobs, dblk_first, (uint32_t)n, S.blk_chunks, lens, st) == hipSuccess;
+  good = good && hipMemcpyAsync(D.hbuf, dres, S.rbytes, hipMemcpyDeviceToHost, st) == hipSuccess;
+  if (!good) (void)hipGetLastError();
+  return good;
+}
+
+// After the device work: each array's cursor (and index record) from the
+// downloaded results, and its size; returns the arrays that succeeded.
+size_t batch_finish(const BatchSet &S, const BatchDev &D, zfp_stream *const *zs, size_t *sizes)
+{
+  size_t ok = 0;
+  const uint64_t *res = (const uint64_t *)D.hbuf;
+  for (size_t k = 0; k < S.items.size(); k++) {
+    const BatchItem &it = S.items[k];
+    zfp_stream *z = zs[it.input];
+    bitstream *s = z->stream;
+    const uint64_t end = 64 * it.w0 + res[2 * k];
+    const uint64_t w1 = (end + 63) / 64;
+    if (S.encode) {
+      // the cursor after the last block, then flush (core.rs:536-542)
+      if (w1 > s->nwords) s->overflowed = true;
+      s->word_pos = (size_t)w1;
+      s->bits = 0;
+      s->buffer = 0;
+      if (!it.job.fixed && s->index) {
+        s->index_n = it.plan.nblocks;
+        s->index_start = it.start;
+        s->index_key = cfg_key(cfg_of(z), it.plan);
+      }
+      if (!s->overflowed) {
+        sizes[it.input] = s->word_pos * 8;
+        ok++;
+      }
+    } else {
+      const uint64_t loaded_end = it.w0 + res[2 * k + 1];
+      s->word_pos = (size_t)w1;
+      s->bits = 0;
+      s->buffer = 0;
+      s->seek_word = s->word_pos;
+      s->overread = loaded_end > std::max<uint64_t>(s->nwords, it.seek_word0);
+      if (!s->overread_now()) {
+        sizes[it.input] = s->word_pos * 8;
+        ok++;
       }
     }
   }
-  std::sort(later.begin(), later.end());
-  for (size_t i : later) {
+  return ok;
+}
+
+size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_t count, size_t *sizes,
+                 uint flags)
+{
+  if (!zs || !fs || !sizes) return 0;
+  size_t ok = 0;
+  BatchSet S;
+  batch_prepare(encode, zs, fs, count, sizes, flags, S);
+  if (!S.items.empty()) {
+    const hipStream_t st = hip_stream_of(zs[S.items[0].input]);
+    Workspace &W = ws();
+    BatchBuffers &B = batch_buffers();
+    std::lock_guard<std::mutex> lock(W.mu);
+    bool good = grow_pinned(B.hbuf, B.hcap, batch_host_bytes(S)) &&
+                Workspace::grow(B.dbuf, B.dcap, batch_dev_bytes(S));
+    if (good && S.var_total) {
+      good = Workspace::grow(W.lens, W.lens_cap, S.var_total * 4) &&
+             Workspace::grow(W.offs, W.offs_cap, (S.var_total + 1) * 8) &&
+             Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(S.var_total) * 8);
+    }
+    BatchDev D;
+    D.hbuf = B.hbuf;
+    D.dbuf = B.dbuf;
+    D.lens = W.lens;
+    D.offs = W.offs;
+    D.tiles = W.tiles;
+    if (good) {
+      batch_tables(S, D);
+      good = batch_enqueue(S, D, true, st) && hipStreamSynchronize(st) == hipSuccess;
+      if (!good) (void)hipGetLastError();
+    }
+    if (good) ok += batch_finish(S, D, zs, sizes);
+  }
+  std::sort(S.later.begin(), S.later.end());
+  for (size_t i : S.later) {
     sizes[i] = encode ? zfp_compress(zs[i], fs[i]) : zfp_decompress(zs[i], fs[i]);
     if (sizes[i]) ok++;
   }
   return ok;
+}
+
+// Cursor state of a stream that a planned batch restores before each run.
+struct SavedCursor {
+  size_t word_pos;
+  uint64_t buffer;
+  uint32_t bits;
+  size_t seek_word;
+};
+
+SavedCursor save_cursor(const bitstream *s) { return {s->word_pos, s->buffer, s->bits, s->seek_word}; }
+
+void restore_cursor(bitstream *s, const SavedCursor &c)
+{
+  s->word_pos = c.word_pos;
+  s->buffer = c.buffer;
+  s->bits = c.bits;
+  s->seek_word = c.seek_word;
+}
+
+bool same_field(const zfp_field &a, const zfp_field &b)
+{
+  return a.type == b.type && a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.nw == b.nw &&
+         a.sx == b.sx && a.sy == b.sy && a.sz == b.sz && a.sw == b.sw && a.data == b.data;
+}
+
+bool same_cfg(const zx::Cfg &a, const zx::Cfg &b)
+{
+  return a.minbits == b.minbits && a.maxbits == b.maxbits && a.maxprec == b.maxprec &&
+         a.minexp == b.minexp && a.rounding == b.rounding && a.tight == b.tight;
+}
+
+}  // namespace
+
+// A batch planned once and replayed from a captured HIP graph.
+struct zfp_hip_batch {
+  BatchSet S;
+  BatchDev D;
+  std::vector<zfp_stream *> zs;
+  std::vector<zfp_field> fields;       // the fields as planned
+  std::vector<SavedCursor> cursors;    // per input: the cursor at plan time
+  std::vector<bitstream> streams;      // per input: the stream as planned (words, index)
+  hipStream_t capture = nullptr;       // private stream the graph was captured on
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double tuned_ms = 0;                 // the chosen tiling's time at plan time
+  size_t dcap = 0;
+
+  ~zfp_hip_batch()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture) (void)hipStreamDestroy(capture);
+    if (D.hbuf) (void)hipHostFree(D.hbuf);
+    for (void *p : {D.dbuf, D.lens, D.offs, D.tiles})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+bool dev_alloc(void *&p, size_t bytes)
+{
+  if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    return false;
+  }
+  return true;
+}
+
+// Time of one run of the batch's device work with the current tiling
+// (best of `reps`), or a negative value when a launch fails.
+double time_batch(zfp_hip_batch &P, hipStream_t st, int reps)
+{
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return -1;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return -1;
+  }
+  double best = -1;
+  for (int r = 0; r < reps; r++) {
+    float ms = 0;
+    const bool good = hipEventRecord(a, st) == hipSuccess && batch_enqueue(P.S, P.D, false, st) &&
+                      hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+                      hipEventElapsedTime(&ms, a, b) == hipSuccess;
+    if (!good) {
+      (void)hipGetLastError();
+      best = -1;
+      break;
+    }
+    if (best < 0 || ms < best) best = ms;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return best;
+}
+
+// Slices of the LDS path every array launched alone can take.
+bool slice_fits(const BatchSet &S, uint32_t T)
+{
+  for (const BatchItem &it : S.items) {
+    if (it.cls < kSoloClass) continue;
+    zx::FxPlan p;
+    if (!zx::fx_plan(it.job.fixed, it.job.start, !S.encode, T, p)) return false;
+  }
+  return true;
 }
 
 }  // namespace
@@ -1962,6 +2138,148 @@ size_t zfp_hip_decompress_batch(zfp_stream *const *streams, zfp_field *const *fi
 {
   return batch_run(false, streams, fields, count, sizes, flags);
 }
+
+zfp_hip_batch *zfp_hip_batch_plan(int encode, zfp_stream *const *streams, zfp_field *const *fields,
+                                  size_t count, uint flags)
+{
+  if (!streams || !fields || count == 0) return nullptr;
+  std::unique_ptr<zfp_hip_batch> P(new zfp_hip_batch);
+  std::vector<SavedCursor> before(count);
+  for (size_t i = 0; i < count; i++) {
+    if (!streams[i] || !streams[i]->stream || !fields[i]) return nullptr;
+    before[i] = save_cursor(streams[i]->stream);
+  }
+  std::vector<size_t> sizes(count);
+  batch_prepare(encode != 0, streams, fields, count, sizes.data(), flags, P->S);
+  // every array must be one the batched launches take
+  if (!P->S.later.empty() || P->S.items.size() != count) {
+    for (size_t i = 0; i < count; i++)
+      if (streams[i] && streams[i]->stream) restore_cursor(streams[i]->stream, before[i]);
+    return nullptr;
+  }
+  P->zs.assign(streams, streams + count);
+  for (size_t i = 0; i < count; i++) {
+    P->fields.push_back(*fields[i]);
+    P->cursors.push_back(before[i]);
+    P->streams.push_back(*streams[i]->stream);
+  }
+  BatchSet &S = P->S;
+  BatchDev &D = P->D;
+  if (hipHostMalloc(&D.hbuf, batch_host_bytes(S), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    D.hbuf = nullptr;
+    return nullptr;
+  }
+  if (!dev_alloc(D.dbuf, batch_dev_bytes(S))) return nullptr;
+  if (S.var_total && !(dev_alloc(D.lens, S.var_total * 4) && dev_alloc(D.offs, (S.var_total + 1) * 8) &&
+                       dev_alloc(D.tiles, zx_scan_tiles(S.var_total) * 8)))
+    return nullptr;
+  const hipStream_t st = hip_stream_of(streams[S.items[0].input]);
+  batch_tables(S, D);
+  if (hipMemcpyAsync(D.dbuf, D.hbuf, S.jbytes + S.fbytes, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  // The tuner: blocks per workgroup of the class launches, then the slice of
+  // arrays launched alone, each the fastest of a few timed runs of the whole
+  // batch (runs that write what a run writes).
+  const bool tune = !(flags & ZFP_HIP_BATCH_NO_TUNE);
+  double best = -1;
+  if (tune) {
+    uint32_t pick = D.threads;
+    for (uint32_t T : {64u, 128u, 256u}) {
+      D.threads = T;
+      const double ms = time_batch(*P, st, 2);
+      if (ms < 0) return nullptr;
+      if (best < 0 || ms < best) {
+        best = ms;
+        pick = T;
+      }
+    }
+    D.threads = pick;
+    bool solo = false;
+    for (const BatchItem &it : S.items) solo = solo || it.cls >= kSoloClass;
+    if (solo) {
+      uint32_t spick = 0;
+      for (uint32_t T : {64u, 128u, 256u}) {
+        if (!slice_fits(S, T)) continue;
+        D.slice = T;
+        const double ms = time_batch(*P, st, 2);
+        if (ms < 0) return nullptr;
+        if (ms < best) {
+          best = ms;
+          spick = T;
+        }
+      }
+      D.slice = spick;
+    }
+  }
+  P->tuned_ms = best;
+  // capture the device work once, on a private stream
+  if (hipStreamCreateWithFlags(&P->capture, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    P->capture = nullptr;
+    return nullptr;
+  }
+  if (hipStreamBeginCapture(P->capture, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  const bool rec = batch_enqueue(S, D, false, P->capture);
+  const hipError_t e = hipStreamEndCapture(P->capture, &P->graph);
+  if (!rec || e != hipSuccess || !P->graph) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    P->exec = nullptr;
+    return nullptr;
+  }
+  for (size_t i = 0; i < count; i++) restore_cursor(streams[i]->stream, before[i]);
+  return P.release();
+}
+
+size_t zfp_hip_batch_run(zfp_hip_batch *P, size_t *sizes)
+{
+  if (!P || !sizes) return 0;
+  const size_t count = P->zs.size();
+  for (size_t i = 0; i < count; i++) sizes[i] = 0;
+  // the arrays, streams and configurations must be the ones planned
+  for (const BatchItem &it : P->S.items) {
+    const zfp_stream *z = P->zs[it.input];
+    const bitstream *s = z->stream;
+    const bitstream &was = P->streams[it.input];
+    if (!s || s->words != was.words || s->nwords != was.nwords) return 0;
+    if (!same_cfg(cfg_of(z), it.job.cfg)) return 0;
+    if (it.job.index_out && (s->index != it.job.index_out || s->index_cap < it.plan.nblocks)) return 0;
+    if (it.job.index && s->index != it.job.index) return 0;
+  }
+  for (const BatchItem &it : P->S.items) {
+    bitstream *s = P->zs[it.input]->stream;
+    restore_cursor(s, P->cursors[it.input]);
+    if (!P->S.encode) s->reset_overread();
+  }
+  const hipStream_t st = hip_stream_of(P->zs[P->S.items[0].input]);
+  if (hipGraphLaunch(P->exec, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return batch_finish(P->S, P->D, P->zs.data(), sizes);
+}
+
+size_t zfp_hip_batch_count(const zfp_hip_batch *P) { return P ? P->zs.size() : 0; }
+
+double zfp_hip_batch_tiling(const zfp_hip_batch *P, uint *threads, uint *slice)
+{
+  if (!P) return -1;
+  if (threads) *threads = P->D.threads;
+  if (slice) *slice = P->D.slice;
+  return P->tuned_ms;
+}
+
+void zfp_hip_batch_free(zfp_hip_batch *P) { delete P; }
 
 int zfp_hip_device(void)
 {

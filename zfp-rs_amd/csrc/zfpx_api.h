@@ -19,6 +19,7 @@ struct EncodeArgs {
   const uint64_t *offsets;  // per-block start bits (variable placement), or null
   uint32_t *lengths;        // non-null: length pass, write bit counts here
   bool vec;                 // rows of whole blocks may be moved as 16-byte vectors
+  uint32_t slice;           // blocks per workgroup of the LDS path, or 0 for its default
 };
 
 struct DecodeArgs {
@@ -31,6 +32,7 @@ struct DecodeArgs {
   const uint64_t *offsets;
   uint64_t *maxword;        // one past the highest word loaded (written by the last block), or null
   bool vec;
+  uint32_t slice;           // blocks per workgroup of the LDS path, or 0 for its default
 };
 
 // One array of a batch (zfp_hip_compress_batch / zfp_hip_decompress_batch).
@@ -60,6 +62,7 @@ struct BatchArgs {
   uint64_t nblocks;
   uint32_t *lengths;        // batch-wide per-block bit lengths (variable placement)
   const uint64_t *offsets;  // batch-wide exclusive scan of `lengths`
+  uint32_t threads;         // workgroup size (blocks per workgroup): 64, 128 or 256
 };
 
 }  // namespace zx

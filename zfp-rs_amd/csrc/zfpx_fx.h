@@ -44,11 +44,15 @@ ZX_HD uint32_t fx_log2(uint32_t v)
   return s;
 }
 
+// LDS a workgroup of the path may use.
+constexpr uint64_t kFxMaxLdsBytes = 64 * 1024;
+
 // Plan of the LDS path for blocks of `fixed` bits from bit `start`; false when
-// blocks are too large for it.  `T` overrides the workgroup size when nonzero.
+// blocks are too large for it.  `T` (at most 256) overrides the workgroup size
+// when nonzero; false when that workgroup's words would not fit in LDS.
 ZX_HD bool fx_plan(uint32_t fixed, uint64_t start, bool decode, uint32_t T, FxPlan &p)
 {
-  if (fixed == 0 || fixed > 64 * 64) return false;
+  if (fixed == 0 || fixed > 64 * 64 || T > 256) return false;
   const uint32_t W = (fixed + 63) / 64;
   p.T = T ? T : W <= 16 ? 256u : W <= 32 ? 128u : 64u;
   p.own = !decode && fixed % 64 == 0 && start % 64 == 0;
@@ -60,7 +64,7 @@ ZX_HD bool fx_plan(uint32_t fixed, uint64_t start, bool decode, uint32_t T, FxPl
     const uint32_t words = (uint32_t)(((uint64_t)p.T * fixed + 63) / 64) + 3;
     p.lds_words = words + (words >> p.skew) + 1;
   }
-  return true;
+  return (uint64_t)p.lds_words * 8 <= kFxMaxLdsBytes;
 }
 
 // One workgroup's share of the stream.

@@ -230,7 +230,7 @@ template <class S, int D, bool REV, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_batch_encode(BatchArgs a)
 {
   expand8_fill();
-  const uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nblocks) return;
   const uint32_t j = job_of(a.firsts, a.njobs, g);
   const BatchJob &J = a.jobs[j];
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_encode(BatchArgs a)
 template <class S, int D, bool REV>
 __global__ __launch_bounds__(kThreads) void k_batch_decode(BatchArgs a)
 {
-  const uint64_t g = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nblocks) return;
   const uint32_t j = job_of(a.firsts, a.njobs, g);
   const BatchJob &J = a.jobs[j];
@@ -295,7 +295,7 @@ static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
 {
   if (a.field.nblocks == 0) return hipSuccess;
   FxPlan p;
-  if (!a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, 0, p)) {
+  if (!a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p)) {
     const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
     const size_t bytes = (size_t)p.lds_words * 8;
     if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
@@ -317,7 +317,7 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
   }
   if (a.field.nblocks == 0) return hipSuccess;
   FxPlan p;
-  if (!a.offsets && fx_plan(a.fixed, a.start, true, 0, p)) {
+  if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p)) {
     const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
     hipLaunchKernelGGL((k_decode_fx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
     return hipGetLastError();
@@ -331,9 +331,10 @@ template <class S, int D, bool REV>
 static hipError_t launch_batch_encode_t(const BatchArgs &a, bool count, hipStream_t s)
 {
   if (a.nblocks == 0) return hipSuccess;
-  const uint64_t grid = (a.nblocks + kThreads - 1) / kThreads;
-  if (count) hipLaunchKernelGGL((k_batch_encode<S, D, REV, true>), dim3(grid), dim3(kThreads), 0, s, a);
-  else hipLaunchKernelGGL((k_batch_encode<S, D, REV, false>), dim3(grid), dim3(kThreads), 0, s, a);
+  const uint32_t T = a.threads ? a.threads : kThreads;
+  const uint64_t grid = (a.nblocks + T - 1) / T;
+  if (count) hipLaunchKernelGGL((k_batch_encode<S, D, REV, true>), dim3(grid), dim3(T), 0, s, a);
+  else hipLaunchKernelGGL((k_batch_encode<S, D, REV, false>), dim3(grid), dim3(T), 0, s, a);
   return hipGetLastError();
 }
 
@@ -346,8 +347,9 @@ static hipError_t launch_batch_decode_t(const BatchArgs &a, bool walk, hipStream
     return hipGetLastError();
   }
   if (a.nblocks == 0) return hipSuccess;
-  const uint64_t grid = (a.nblocks + kThreads - 1) / kThreads;
-  hipLaunchKernelGGL((k_batch_decode<S, D, REV>), dim3(grid), dim3(kThreads), 0, s, a);
+  const uint32_t T = a.threads ? a.threads : kThreads;
+  const uint64_t grid = (a.nblocks + T - 1) / T;
+  hipLaunchKernelGGL((k_batch_decode<S, D, REV>), dim3(grid), dim3(T), 0, s, a);
   return hipGetLastError();
 }
 

@@ -127,8 +127,14 @@ SIGNATURES = {
     "zfp_hip_device": (ctypes.c_int, []),
     "zfp_hip_compress_batch": (_SZ, [P(_SP), P(_FP), _SZ, P(_SZ), _U]),
     "zfp_hip_decompress_batch": (_SZ, [P(_SP), P(_FP), _SZ, P(_SZ), _U]),
+    "zfp_hip_batch_plan": (_VP, [ctypes.c_int, P(_SP), P(_FP), _SZ, _U]),
+    "zfp_hip_batch_run": (_SZ, [_VP, P(_SZ)]),
+    "zfp_hip_batch_count": (_SZ, [_VP]),
+    "zfp_hip_batch_tiling": (_D, [_VP, P(_U), P(_U)]),
+    "zfp_hip_batch_free": (None, [_VP]),
 }
 BATCH_DEVICE_FIELDS = 1
+BATCH_NO_TUNE = 2
 for _name in ("zfp_config_none", "zfp_config_rate", "zfp_config_precision", "zfp_config_accuracy",
               "zfp_config_reversible", "zfp_config_expert"):
     SIGNATURES[_name] = (None, None)  # struct returns: exported, not called from Python
@@ -396,6 +402,50 @@ class Batch:
 
     def decompress(self, flags=BATCH_DEVICE_FIELDS):
         return lib().zfp_hip_decompress_batch(self.zs, self.fout, self.n, self.sizes, flags)
+
+
+class BatchPlan:
+    """zfp_hip_batch_plan: a batch planned (and its tiling tuned) once, then
+    replayed from a captured HIP graph by run().  `encode` selects compress
+    (arrays -> streams) or decompress (streams -> arrays); the streams' cursors
+    at construction are where every run starts."""
+
+    def __init__(self, encode, streams, arrays, flags=BATCH_DEVICE_FIELDS):
+        n = len(streams)
+        if len(arrays) != n:
+            raise ValueError("one array per stream")
+        self.n = n
+        self._keep = (streams, arrays)
+        self._fields = [make_field(a) for a in arrays]
+        zs = (_SP * n)(*[s.zs for s in streams])
+        fps = (_FP * n)(*[ctypes.pointer(f) for f in self._fields])
+        self.sizes = (ctypes.c_size_t * n)()
+        self.handle = lib().zfp_hip_batch_plan(1 if encode else 0, zs, fps, n, flags)
+        if not self.handle:
+            raise ValueError("batch not plannable: every array must be device-resident, "
+                             "one per stream, with at least one block")
+
+    def run(self):
+        """Replay; returns (arrays coded, per-array byte counts)."""
+        done = lib().zfp_hip_batch_run(self.handle, self.sizes)
+        return done, list(self.sizes)
+
+    def tiling(self):
+        """(blocks per workgroup, LDS slice or 0, tuned ms or -1)."""
+        t, s = ctypes.c_uint(0), ctypes.c_uint(0)
+        ms = lib().zfp_hip_batch_tiling(self.handle, ctypes.byref(t), ctypes.byref(s))
+        return t.value, s.value, ms
+
+    def close(self):
+        if self.handle:
+            lib().zfp_hip_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def compress_batch(streams, arrays, flags=0):
