@@ -93,7 +93,9 @@ def batched_leg(n_arrays, reps, seed=4096):
     its own seed with the k^-3 field), device-resident.  One step = compress
     then decompress every array.  Times the batched entry points against one
     zfp_compress / zfp_decompress call per array, checks that both give the
-    same bytes, and reports the batched step's p50/p99 over `reps` steps."""
+    same bytes, and reports the batched step's p50/p99 over `reps` steps; then
+    the same batch planned once (zfp_hip_batch_plan: tiling tuned, device
+    work captured into a HIP graph) and replayed, with its own p50/p99."""
     import math
 
     import numpy as np
@@ -191,11 +193,49 @@ def batched_leg(n_arrays, reps, seed=4096):
     batch.decompress()
     torch.cuda.synchronize()
     same = same and all(outs[k].cpu().equal(ref_out[k]) for k in sample)
+
+    # the same batch planned once (tiling tuned by timing it) and replayed
+    # from captured graphs, one for compress and one for decompress
+    rewind()
+    t0 = time.perf_counter()
+    enc_plan = Z.BatchPlan(True, streams, arrays)
+    rewind()
+    dec_plan = Z.BatchPlan(False, streams, outs)
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    note(f"planned: encode tiling {enc_plan.tiling()}, decode tiling {dec_plan.tiling()}, "
+         f"{plan_ms:.0f} ms to plan")
+    graph_ms, gcomp_ms, gdecomp_ms = [], [], []
+    gok = True
+    for r in range(reps + 2):
+        for k in sample:
+            bufs[k].zero_()
+            outs[k].zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nc, gsizes = enc_plan.run()
+        t1 = time.perf_counter()
+        nd, _ = dec_plan.run()
+        t2 = time.perf_counter()
+        gok = gok and nc == n_arrays and nd == n_arrays and gsizes == sizes
+        note(f"graph step {r}: {(t2 - t0) * 1e3:.2f} ms")
+        if r >= 2:
+            gcomp_ms.append((t1 - t0) * 1e3)
+            gdecomp_ms.append((t2 - t1) * 1e3)
+            graph_ms.append((t2 - t0) * 1e3)
+    torch.cuda.synchronize()
+    gsame = gok and all(bufs[k][: sizes[k] // 8].cpu().equal(ref_words[k]) for k in sample)
+    gsame = gsame and all(outs[k].cpu().equal(ref_out[k]) for k in sample)
+    et, es, _ = enc_plan.tiling()
+    dt_, ds, _ = dec_plan.tiling()
+    enc_plan.close()
+    dec_plan.close()
     for st in streams:
         st.close()
     best_single = min(single_ms)
     p50 = float(np.percentile(step_ms, 50))
     p99 = float(np.percentile(step_ms, 99))
+    g50 = float(np.percentile(graph_ms, 50))
+    g99 = float(np.percentile(graph_ms, 99))
     res = {
         "arrays": n_arrays, "classes": len(classes), "uncompressed_bytes": field_bytes,
         "compressed_bytes": stream_bytes,
@@ -207,6 +247,17 @@ def batched_leg(n_arrays, reps, seed=4096):
         "decompress_p50_ms": round(float(np.percentile(decomp_ms, 50)), 3),
         "per_call_step_ms": round(best_single, 3), "steps": reps,
         "bytes_match_per_call": bool(same),
+        "graph": {
+            "GBps": round(2 * field_bytes / (g50 * 1e-3) / 1e9, 2),
+            "speedup_vs_per_call": round(best_single / g50, 2),
+            "step_p50_ms": round(g50, 3), "step_p99_ms": round(g99, 3),
+            "compress_p50_ms": round(float(np.percentile(gcomp_ms, 50)), 3),
+            "decompress_p50_ms": round(float(np.percentile(gdecomp_ms, 50)), 3),
+            "plan_ms": round(plan_ms, 1),
+            "tiling": {"compress": {"blocks_per_workgroup": et, "lds_slice": es},
+                       "decompress": {"blocks_per_workgroup": dt_, "lds_slice": ds}},
+            "bytes_match_per_call": bool(gsame),
+        },
     }
     del arrays, outs, bufs, batch
     torch.cuda.empty_cache()
