@@ -112,7 +112,8 @@ static uint64_t decompress_fx(const FieldDesc &f, const Cfg &c, const uint64_t *
     for (uint32_t i = 0; i < q.nw + 2; i++) fx_dec_stage(lds.data(), words, nwords, p, q, i);
     for (uint32_t t = 0; t < q.nb; t++) {
       U v[Blk<D>::SIZE];
-      const uint64_t loaded = fx_dec_block<S, D, REV>(lds.data(), p, q, t, fixed, v, c);
+      const uint64_t loaded = p.flat ? fx_dec_block<S, D, REV, true>(lds.data(), p, q, t, fixed, v, c)
+                                     : fx_dec_block<S, D, REV, false>(lds.data(), p, q, t, fixed, v, c);
       if (q.b0 + t == f.nblocks - 1) maxword = loaded;
       scatter<S, D>(f, block_pos<D>(f, q.b0 + t), v);
     }

@@ -568,16 +568,20 @@ struct GlobalWords {
   uint64_t n;
   ZX_HD uint64_t operator()(uint64_t i) const { return i < n ? w[i] : 0ull; }
 };
-// Word i at slot i + (i >> skew); the pad slot before each word whose index
-// is a multiple of 2^skew holds a second copy of that word, so the two slots
-// from word i's on hold words i and i + 1.
+// A staged copy in LDS.  Word i sits at slot i + (i >> skew), and the pad
+// slot before each word whose index is a multiple of 2^skew holds a second
+// copy of that word, so the two slots from word i's on hold words i and i + 1.
+// FLAT: `w` is the slot of the reader's first word, and its words occupy the
+// slots after it with no pad between them (a block of whole words that no
+// multiple of 2^skew splits); the pad copy covers the word after the last.
+template <bool FLAT>
 struct StagedWords {
   const uint64_t *w;
   uint32_t skew;
   // words i and i + 1 for the word i holding bit p
   ZX_HD void pair_at(uint32_t p, uint64_t &lo, uint64_t &hi) const
   {
-    const uint64_t *q = w + ((p >> 6) + (p >> (skew + 6)));
+    const uint64_t *q = FLAT ? w + (p >> 6) : w + ((p >> 6) + (p >> (skew + 6)));
     lo = q[0];
     hi = q[1];
   }
@@ -638,12 +642,12 @@ struct Source {
 // Over a staged copy in LDS the reader keeps only its bit position: each peek
 // loads the two words under it, so a step is one add and no refill branch.
 // words_loaded() is the reference cursor's word_pos, ceil(pos / 64).
-template <>
-struct Source<StagedWords> {
-  StagedWords mem;
+template <bool FLAT>
+struct Source<StagedWords<FLAT>> {
+  StagedWords<FLAT> mem;
   uint32_t p;
 
-  ZX_HD void init(const StagedWords &m, uint64_t bitpos)
+  ZX_HD void init(const StagedWords<FLAT> &m, uint64_t bitpos)
   {
     mem = m;
     p = (uint32_t)bitpos;

@@ -34,6 +34,9 @@ struct FxPlan {
   uint32_t T;          // blocks (threads) per workgroup
   uint32_t skew;       // contiguous layouts: word i at LDS slot i + (i >> skew)
   bool own;            // encode layout OWN (see above)
+  bool flat;           // decode: blocks are whole words from a word boundary and
+                       // 2^skew is a multiple of their word count, so no pad slot
+                       // splits a block (readers address their words directly)
   uint32_t lds_words;  // LDS words per workgroup
 };
 
@@ -58,6 +61,7 @@ ZX_HD bool fx_plan(uint32_t fixed, uint64_t start, bool decode, uint32_t T, FxPl
   p.own = !decode && fixed % 64 == 0 && start % 64 == 0;
   const uint32_t s = fx_log2(W);
   p.skew = p.own ? 31u : s < 3 ? 3u : s;
+  p.flat = decode && fixed % 64 == 0 && start % 64 == 0 && ((1u << p.skew) % W) == 0;
   if (p.own) {
     p.lds_words = p.T * (W + 1);
   } else {
@@ -167,16 +171,25 @@ ZX_HD void fx_dec_stage(uint64_t *lds, const uint64_t *words, uint64_t nwords, c
 
 // phase 2: thread t decodes its block into v; returns one past the highest
 // stream word the reference's reader would have loaded for it
-template <class S, int D, bool REV>
+template <class S, int D, bool REV, bool FLAT>
 ZX_HD uint64_t fx_dec_block(const uint64_t *lds, const FxPlan &p, const FxGroup &q, uint32_t t,
                             uint32_t fixed, typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE],
                             const Cfg &c)
 {
   const uint64_t pos = q.r0 + (uint64_t)t * fixed;
-  Source<StagedWords> r;
-  r.init(StagedWords{lds, p.skew}, pos);
-  decode_values<S, D, REV>(r, v, c);
-  return q.w0 + loaded_words(r, pos, c.minbits);
+  if constexpr (FLAT) {
+    // positions relative to the block's first word (q.r0 is 0)
+    const uint32_t i0 = (uint32_t)(pos >> 6);
+    Source<StagedWords<true>> r;
+    r.init(StagedWords<true>{lds + fx_slot(i0, p.skew), p.skew}, 0);
+    decode_values<S, D, REV>(r, v, c);
+    return q.w0 + i0 + loaded_words(r, 0, c.minbits);
+  } else {
+    Source<StagedWords<false>> r;
+    r.init(StagedWords<false>{lds, p.skew}, pos);
+    decode_values<S, D, REV>(r, v, c);
+    return q.w0 + loaded_words(r, pos, c.minbits);
+  }
 }
 
 }  // namespace zx

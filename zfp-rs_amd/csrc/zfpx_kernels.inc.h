@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void k_encode_fx(EncodeArgs a, FxPlan p)
 // Fixed-rate decode through LDS: stage the group's words, then decode.  Held
 // to four waves per SIMD (128 VGPRs, a few spilled): the group-test loop waits
 // on LDS reads, and the extra wave hides more of that than the spills cost.
-template <class S, int D, bool REV>
+template <class S, int D, bool REV, bool FLAT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
   __syncthreads();
   if (t >= q.nb) return;
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
-  const uint64_t loaded = fx_dec_block<S, D, REV>(lds, p, q, t, a.fixed, v, a.cfg);
+  const uint64_t loaded = fx_dec_block<S, D, REV, FLAT>(lds, p, q, t, a.fixed, v, a.cfg);
   // blocks are contiguous, so the last one loads the highest word
   if (a.maxword && q.b0 + t == a.field.nblocks - 1) *a.maxword = loaded;
   store_block<S, D>(a.field, q.b0 + t, a.vec, v);
@@ -319,7 +319,8 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
   FxPlan p;
   if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p)) {
     const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
-    hipLaunchKernelGGL((k_decode_fx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+    if (p.flat) hipLaunchKernelGGL((k_decode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+    else hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
     return hipGetLastError();
   }
   const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
