@@ -510,7 +510,12 @@ struct LdsSink {
     bits = bitpos & 63;
     buf = 0;
   }
-  ZX_HD uint64_t *at(uint32_t i) const { return base + i + (i >> skew); }
+  ZX_HD uint64_t *at(uint32_t i) const
+  {
+    // an owned slice has no pad slots between a block's words
+    if constexpr (SHARED) return base + i + (i >> skew);
+    else return base + i;
+  }
   ZX_HD void store(uint32_t i, uint64_t v)
   {
     if constexpr (SHARED) {
@@ -697,10 +702,11 @@ ZX_HD constexpr Expand8Table make_expand8_table()
 static constexpr Expand8Table kExpand8Host = make_expand8_table();
 
 #if ZX_DEVICE_BUILD
-// The table in device memory, and the workgroup's copy in LDS, which kernels
-// that encode fill (expand8_fill) before any thread codes a block.
+// The table in device memory, and the workgroup's copy in LDS (32-bit
+// entries: a plain dword load needs no masking), which kernels that encode
+// fill (expand8_fill) before any thread codes a block.
 static __constant__ Expand8Table kExpand8Dev = make_expand8_table();
-static __shared__ uint16_t zx_expand8_lds[256];
+static __shared__ uint32_t zx_expand8_lds[256];
 #endif
 
 ZX_HD uint32_t expand8_of(uint32_t b)
