@@ -419,7 +419,7 @@ def variable_rate_leg(steps=5):
 
 
 KERNELS = {"encode": "zx::k_encode_fx<float, 3, false, true>",
-           "decode": "zx::k_decode_fx<float, 3, false>"}
+           "decode": "zx::k_decode_fx<float, 3, false, true>"}
 
 
 def measured_traffic(which, n, rate):
