@@ -202,8 +202,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
   const uint64_t loaded = fx_dec_block<S, D, REV, FLAT>(lds, p, q, t, a.fixed, v, a.cfg);
   // blocks are contiguous, so the last one loads the highest word
-  if (a.maxword && q.b0 + t == a.field.nblocks - 1) *a.maxword = loaded;
-  store_block<S, D>(a.field, q.b0 + t, a.vec, v);
+  // block indices fit in 32 bits (the host caps nblocks below 2^32)
+  const uint32_t b = (uint32_t)q.b0 + t;
+  if (a.maxword && b == (uint32_t)(a.field.nblocks - 1)) *a.maxword = loaded;
+  store_block<S, D>(a.field, b, a.vec, v);
 }
 
 // ------------------------------------------------------------------ batches
