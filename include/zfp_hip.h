@@ -94,6 +94,22 @@ size_t zfp_hip_batch_count(const zfp_hip_batch *batch);
 double zfp_hip_batch_tiling(const zfp_hip_batch *batch, uint *threads, uint *slice);
 void zfp_hip_batch_free(zfp_hip_batch *batch);
 
+/* Asynchronous batches, for serving loops that keep several batches in
+ * flight.  zfp_hip_batch_submit: plans the arrays as zfp_hip_batch_plan does
+ * (the same conditions; no tuning, no graph), enqueues their device work on
+ * `hip_stream` (a hipStream_t; NULL: the first stream's) and returns without
+ * waiting; NULL when it cannot take the arrays (cursors unchanged).  The
+ * streams and arrays must not be used until the wait.  zfp_hip_batch_wait:
+ * waits for that work, leaves cursors and sizes as the batch call would,
+ * frees the submission and returns the arrays coded.  Submissions on
+ * different HIP streams overlap on the device.  No counterpart in the
+ * reference (an extension beside zfp_hip_compress_batch). */
+zfp_hip_batch *zfp_hip_batch_submit(int encode, zfp_stream *const *streams, zfp_field *const *fields,
+                                    size_t count, uint flags, void *hip_stream);
+size_t zfp_hip_batch_wait(zfp_hip_batch *submission, size_t *sizes);
+/* zfp_true when a submission's device work has completed (wait will not block). */
+zfp_bool zfp_hip_batch_ready(const zfp_hip_batch *submission);
+
 /* Device the library runs on (the current HIP device when first used). */
 int zfp_hip_device(void);
 

@@ -53,8 +53,11 @@ def make_requests(n=40, seed=3):
     return reqs, refs
 
 
+@pytest.mark.parametrize("inflight", [1, 2])
 @pytest.mark.parametrize("rate", [2000.0, 0.0])
-def test_each_request_is_the_same_alone_and_in_the_mix(rate):
+def test_each_request_is_the_same_alone_and_in_the_mix(rate, inflight):
+    """One batch at a time, and two batches in flight on separate HIP streams
+    (submitted without waiting, overlapping on the device)."""
     reqs, refs = make_requests()
     rng = np.random.default_rng(9)
     t = 0.0
@@ -62,7 +65,7 @@ def test_each_request_is_the_same_alone_and_in_the_mix(rate):
         if rate:
             t += rng.exponential(1.0 / rate)
         r.arrival = t
-    stats = serve.serve_batched(reqs, serve.GpuCodec(), budget=1 << 40)
+    stats = serve.serve_batched(reqs, serve.GpuCodec(), budget=1 << 40, inflight=inflight)
     torch.cuda.synchronize()
     assert stats["batches"] >= 1
     for r, (kind, n_ref, want, buf) in zip(reqs, refs):
@@ -73,5 +76,25 @@ def test_each_request_is_the_same_alone_and_in_the_mix(rate):
         else:
             assert r.size == n_ref
             assert torch.equal(r.array, want)
+    for r in reqs:
+        r.stream.close()
+
+
+def test_submissions_on_two_streams_match_single_calls():
+    """zfp_hip_batch_submit / zfp_hip_batch_wait directly: two batches in
+    flight at once, each array's bytes and size equal to its single call."""
+    reqs, refs = make_requests(n=24, seed=8)
+    comp = [(r, ref) for r, ref in zip(reqs, refs) if r.kind == "compress"]
+    codec = serve.GpuCodec(slots=2)
+    half = len(comp) // 2
+    t1 = codec.submit([r for r, _ in comp[:half]], [], 0)
+    t2 = codec.submit([r for r, _ in comp[half:]], [], 1)
+    assert len(t1) == 1 and len(t2) == 1  # both taken by the asynchronous path
+    codec.wait(t2)
+    codec.wait(t1)
+    torch.cuda.synchronize()
+    for r, (kind, n_ref, want, buf) in comp:
+        assert r.size == n_ref
+        assert torch.equal(buf[: n_ref // 8], want)
     for r in reqs:
         r.stream.close()

@@ -1418,9 +1418,33 @@ bool same_cfg(const zx::Cfg &a, const zx::Cfg &b)
          a.minexp == b.minexp && a.rounding == b.rounding && a.tight == b.tight;
 }
 
+// Buffers of an asynchronous submission (pinned tables and results, their
+// device copy, the variable-rate scratch), reused across submissions: a
+// submission takes a set from the pool and its wait returns it.
+struct SubmitBuffers {
+  void *hbuf = nullptr; size_t hcap = 0;
+  void *dbuf = nullptr; size_t dcap = 0;
+  void *lens = nullptr; size_t lens_cap = 0;
+  void *offs = nullptr; size_t offs_cap = 0;
+  void *tiles = nullptr; size_t tiles_cap = 0;
+};
+
+struct SubmitPool {
+  std::mutex mu;
+  std::vector<SubmitBuffers *> free;
+};
+
+SubmitPool &submit_pool()
+{
+  static SubmitPool p;
+  return p;
+}
+
 }  // namespace
 
-// A batch planned once and replayed from a captured HIP graph.
+// A batch planned once and replayed from a captured HIP graph
+// (zfp_hip_batch_plan), or submitted once without waiting
+// (zfp_hip_batch_submit).
 struct zfp_hip_batch {
   BatchSet S;
   BatchDev D;
@@ -1432,13 +1456,21 @@ struct zfp_hip_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double tuned_ms = 0;                 // the chosen tiling's time at plan time
-  size_t dcap = 0;
+  SubmitBuffers *pooled = nullptr;     // a submission's buffers (returned to the pool)
+  hipEvent_t done = nullptr;           // a submission's completion
 
   ~zfp_hip_batch()
   {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture) (void)hipStreamDestroy(capture);
+    if (done) (void)hipEventDestroy(done);
+    if (pooled) {
+      SubmitPool &pool = submit_pool();
+      std::lock_guard<std::mutex> lock(pool.mu);
+      pool.free.push_back(pooled);
+      return;
+    }
     if (D.hbuf) (void)hipHostFree(D.hbuf);
     for (void *p : {D.dbuf, D.lens, D.offs, D.tiles})
       if (p) (void)hipFree(p);
@@ -1455,6 +1487,74 @@ bool dev_alloc(void *&p, size_t bytes)
     return false;
   }
   return true;
+}
+
+// Plans a batch whose every array the batched launches take (else null,
+// cursors left as they were) and writes its tables into its pinned buffer:
+// buffers owned by the batch, or (pooled) a set from the submission pool.
+std::unique_ptr<zfp_hip_batch> make_batch(bool encode, zfp_stream *const *streams,
+                                          zfp_field *const *fields, size_t count, uint flags,
+                                          bool pooled)
+{
+  if (!streams || !fields || count == 0) return nullptr;
+  std::unique_ptr<zfp_hip_batch> P(new zfp_hip_batch);
+  std::vector<SavedCursor> before(count);
+  for (size_t i = 0; i < count; i++) {
+    if (!streams[i] || !streams[i]->stream || !fields[i]) return nullptr;
+    before[i] = save_cursor(streams[i]->stream);
+  }
+  std::vector<size_t> sizes(count);
+  batch_prepare(encode, streams, fields, count, sizes.data(), flags, P->S);
+  // every array must be one the batched launches take
+  if (!P->S.later.empty() || P->S.items.size() != count) {
+    for (size_t i = 0; i < count; i++) restore_cursor(streams[i]->stream, before[i]);
+    return nullptr;
+  }
+  P->zs.assign(streams, streams + count);
+  for (size_t i = 0; i < count; i++) {
+    P->fields.push_back(*fields[i]);
+    P->cursors.push_back(before[i]);
+    P->streams.push_back(*streams[i]->stream);
+  }
+  BatchSet &S = P->S;
+  BatchDev &D = P->D;
+  if (pooled) {
+    SubmitPool &pool = submit_pool();
+    SubmitBuffers *b = nullptr;
+    {
+      std::lock_guard<std::mutex> lock(pool.mu);
+      if (!pool.free.empty()) {
+        b = pool.free.back();
+        pool.free.pop_back();
+      }
+    }
+    if (!b) b = new SubmitBuffers;
+    P->pooled = b;  // back to the pool with the batch, whatever happens next
+    bool good = grow_pinned(b->hbuf, b->hcap, batch_host_bytes(S)) &&
+                Workspace::grow(b->dbuf, b->dcap, batch_dev_bytes(S));
+    if (good && S.var_total)
+      good = Workspace::grow(b->lens, b->lens_cap, S.var_total * 4) &&
+             Workspace::grow(b->offs, b->offs_cap, (S.var_total + 1) * 8) &&
+             Workspace::grow(b->tiles, b->tiles_cap, zx_scan_tiles(S.var_total) * 8);
+    if (!good) return nullptr;
+    D.hbuf = b->hbuf;
+    D.dbuf = b->dbuf;
+    D.lens = b->lens;
+    D.offs = b->offs;
+    D.tiles = b->tiles;
+  } else {
+    if (hipHostMalloc(&D.hbuf, batch_host_bytes(S), hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      D.hbuf = nullptr;
+      return nullptr;
+    }
+    if (!dev_alloc(D.dbuf, batch_dev_bytes(S))) return nullptr;
+    if (S.var_total && !(dev_alloc(D.lens, S.var_total * 4) && dev_alloc(D.offs, (S.var_total + 1) * 8) &&
+                         dev_alloc(D.tiles, zx_scan_tiles(S.var_total) * 8)))
+      return nullptr;
+  }
+  batch_tables(S, D);
+  return P;
 }
 
 // Time of one run of the batch's device work with the current tiling
@@ -2142,40 +2242,11 @@ size_t zfp_hip_decompress_batch(zfp_stream *const *streams, zfp_field *const *fi
 zfp_hip_batch *zfp_hip_batch_plan(int encode, zfp_stream *const *streams, zfp_field *const *fields,
                                   size_t count, uint flags)
 {
-  if (!streams || !fields || count == 0) return nullptr;
-  std::unique_ptr<zfp_hip_batch> P(new zfp_hip_batch);
-  std::vector<SavedCursor> before(count);
-  for (size_t i = 0; i < count; i++) {
-    if (!streams[i] || !streams[i]->stream || !fields[i]) return nullptr;
-    before[i] = save_cursor(streams[i]->stream);
-  }
-  std::vector<size_t> sizes(count);
-  batch_prepare(encode != 0, streams, fields, count, sizes.data(), flags, P->S);
-  // every array must be one the batched launches take
-  if (!P->S.later.empty() || P->S.items.size() != count) {
-    for (size_t i = 0; i < count; i++)
-      if (streams[i] && streams[i]->stream) restore_cursor(streams[i]->stream, before[i]);
-    return nullptr;
-  }
-  P->zs.assign(streams, streams + count);
-  for (size_t i = 0; i < count; i++) {
-    P->fields.push_back(*fields[i]);
-    P->cursors.push_back(before[i]);
-    P->streams.push_back(*streams[i]->stream);
-  }
+  std::unique_ptr<zfp_hip_batch> P = make_batch(encode != 0, streams, fields, count, flags, false);
+  if (!P) return nullptr;
   BatchSet &S = P->S;
   BatchDev &D = P->D;
-  if (hipHostMalloc(&D.hbuf, batch_host_bytes(S), hipHostMallocDefault) != hipSuccess) {
-    (void)hipGetLastError();
-    D.hbuf = nullptr;
-    return nullptr;
-  }
-  if (!dev_alloc(D.dbuf, batch_dev_bytes(S))) return nullptr;
-  if (S.var_total && !(dev_alloc(D.lens, S.var_total * 4) && dev_alloc(D.offs, (S.var_total + 1) * 8) &&
-                       dev_alloc(D.tiles, zx_scan_tiles(S.var_total) * 8)))
-    return nullptr;
   const hipStream_t st = hip_stream_of(streams[S.items[0].input]);
-  batch_tables(S, D);
   if (hipMemcpyAsync(D.dbuf, D.hbuf, S.jbytes + S.fbytes, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess) {
     (void)hipGetLastError();
@@ -2237,13 +2308,13 @@ zfp_hip_batch *zfp_hip_batch_plan(int encode, zfp_stream *const *streams, zfp_fi
     P->exec = nullptr;
     return nullptr;
   }
-  for (size_t i = 0; i < count; i++) restore_cursor(streams[i]->stream, before[i]);
+  for (size_t i = 0; i < count; i++) restore_cursor(streams[i]->stream, P->cursors[i]);
   return P.release();
 }
 
 size_t zfp_hip_batch_run(zfp_hip_batch *P, size_t *sizes)
 {
-  if (!P || !sizes) return 0;
+  if (!P || !sizes || !P->exec) return 0;
   const size_t count = P->zs.size();
   for (size_t i = 0; i < count; i++) sizes[i] = 0;
   // the arrays, streams and configurations must be the ones planned
@@ -2280,6 +2351,53 @@ double zfp_hip_batch_tiling(const zfp_hip_batch *P, uint *threads, uint *slice)
 }
 
 void zfp_hip_batch_free(zfp_hip_batch *P) { delete P; }
+
+zfp_hip_batch *zfp_hip_batch_submit(int encode, zfp_stream *const *streams, zfp_field *const *fields,
+                                    size_t count, uint flags, void *hip_stream)
+{
+  std::unique_ptr<zfp_hip_batch> P = make_batch(encode != 0, streams, fields, count, flags, true);
+  if (!P) return nullptr;
+  const hipStream_t st = hip_stream ? (hipStream_t)hip_stream : hip_stream_of(streams[P->S.items[0].input]);
+  const bool good =
+      hipEventCreateWithFlags(&P->done, hipEventDisableTiming) == hipSuccess &&
+      hipMemcpyAsync(P->D.dbuf, P->D.hbuf, P->S.jbytes + P->S.fbytes, hipMemcpyHostToDevice, st) == hipSuccess &&
+      batch_enqueue(P->S, P->D, false, st) && hipEventRecord(P->done, st) == hipSuccess;
+  if (!good) {
+    (void)hipGetLastError();
+    if (P->done) (void)hipEventSynchronize(P->done);
+    (void)hipStreamSynchronize(st);
+    for (size_t i = 0; i < count; i++) restore_cursor(streams[i]->stream, P->cursors[i]);
+    return nullptr;
+  }
+  return P.release();
+}
+
+zfp_bool zfp_hip_batch_ready(const zfp_hip_batch *P)
+{
+  if (!P || !P->done) return zfp_false;
+  const hipError_t e = hipEventQuery(P->done);
+  if (e == hipSuccess) return zfp_true;
+  if (e != hipErrorNotReady) (void)hipGetLastError();
+  return zfp_false;
+}
+
+size_t zfp_hip_batch_wait(zfp_hip_batch *P, size_t *sizes)
+{
+  if (!P || !P->done) return 0;  // (a planned batch is not a submission)
+  std::unique_ptr<zfp_hip_batch> hold(P);
+  const size_t count = P->zs.size();
+  std::vector<size_t> scratch;
+  if (!sizes) {
+    scratch.resize(count);
+    sizes = scratch.data();
+  }
+  for (size_t i = 0; i < count; i++) sizes[i] = 0;
+  if (hipEventSynchronize(P->done) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return batch_finish(P->S, P->D, P->zs.data(), sizes);
+}
 
 int zfp_hip_device(void)
 {

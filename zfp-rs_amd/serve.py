@@ -9,6 +9,12 @@ completion.  Requests that arrive while a batch is on the GPU join the next
 batch as soon as it returns, so the device always works on whatever has
 arrived; a request too large to share the budget runs alone.
 
+With `inflight` > 1 and a codec that can submit without waiting, the loop
+keeps up to that many batches on the device at once (each on its own HIP
+stream): requests that arrive while a batch runs are admitted and submitted
+as soon as a slot is free and their bytes fit beside the batches in flight,
+so their work overlaps the running batch instead of queueing behind it.
+
 `serve_one_at_a_time` replays the same trace with one zfp_compress /
 zfp_decompress per request, the path the batched loop is compared with.
 """
@@ -36,11 +42,43 @@ class Request:
 
 
 class GpuCodec:
-    """The batched entry points of libzfp_hip.so."""
+    """The batched entry points of libzfp_hip.so; `slots` HIP streams for
+    batches in flight (zfp_hip_batch_submit / zfp_hip_batch_wait)."""
 
-    def __init__(self):
+    def __init__(self, slots=2):
+        import torch
+
         import zfp_hip as Z
         self.Z = Z
+        self.streams = [torch.cuda.Stream() for _ in range(slots)]
+
+    def submit(self, compress, decompress, slot):
+        """Enqueue the batch's compress and decompress on slot `slot`'s
+        stream; a group the asynchronous path cannot take runs at once
+        through the batch call.  Returns what wait() needs."""
+        Z = self.Z
+        hs = self.streams[slot % len(self.streams)].cuda_stream
+        ticket = []
+        for reqs, encode in ((compress, True), (decompress, False)):
+            if not reqs:
+                continue
+            b = Z.Batch([r.stream for r in reqs], [r.array for r in reqs])
+            h = Z.lib().zfp_hip_batch_submit(1 if encode else 0, b.zs, b.fin, b.n,
+                                             Z.BATCH_DEVICE_FIELDS, hs)
+            if h:
+                ticket.append((h, reqs, b))
+            else:
+                self.run(reqs if encode else [], [] if encode else reqs)
+        return ticket
+
+    def ready(self, ticket):
+        return all(self.Z.lib().zfp_hip_batch_ready(h) for h, _, _ in ticket)
+
+    def wait(self, ticket):
+        for h, reqs, b in ticket:
+            self.Z.lib().zfp_hip_batch_wait(h, b.sizes)
+            for r, n in zip(reqs, b.sizes):
+                r.size = n
 
     def run(self, compress, decompress):
         Z = self.Z
@@ -73,9 +111,11 @@ def admit(pending, budget):
     return batch, used
 
 
-def serve_batched(trace, codec, budget, clock=time.perf_counter, sleep=time.sleep):
+def serve_batched(trace, codec, budget, clock=time.perf_counter, sleep=time.sleep, inflight=1):
     """Replay `trace` (requests sorted by arrival) through the batching loop.
     Returns per-batch statistics; each request gets `done` and `size`."""
+    if inflight > 1 and hasattr(codec, "submit"):
+        return _serve_overlapped(trace, codec, budget, clock, sleep, inflight)
     pending = collections.deque()
     todo = collections.deque(trace)
     t0 = clock()
@@ -96,6 +136,53 @@ def serve_batched(trace, codec, budget, clock=time.perf_counter, sleep=time.slee
         stats["batches"] += 1
         stats["max_batch"] = max(stats["max_batch"], len(batch))
         stats["peak_bytes"] = max(stats["peak_bytes"], used)
+    return stats
+
+
+def _serve_overlapped(trace, codec, budget, clock, sleep, inflight, poll=20e-6):
+    """The loop with up to `inflight` batches on the device: a new batch is
+    submitted whenever a slot is free and the first waiting request fits in
+    the budget beside the batches in flight (or nothing is in flight);
+    otherwise the loop waits for the oldest batch."""
+    pending = collections.deque()
+    todo = collections.deque(trace)
+    flying = collections.deque()  # (ticket, batch, bytes)
+    used = 0
+    slot = 0
+    t0 = clock()
+    stats = {"batches": 0, "max_batch": 0, "peak_bytes": 0, "max_inflight": 0, "overlapped": 0}
+    while todo or pending or flying:
+        now = clock() - t0
+        while todo and todo[0].arrival <= now:
+            pending.append(todo.popleft())
+        if pending and len(flying) < inflight and (not flying or pending[0].footprint <= budget - used):
+            batch, nbytes = admit(pending, budget - used if flying else budget)
+            ticket = codec.submit([r for r in batch if r.kind == "compress"],
+                                  [r for r in batch if r.kind == "decompress"], slot)
+            slot = (slot + 1) % inflight
+            stats["overlapped"] += 1 if flying else 0
+            flying.append((ticket, batch, nbytes))
+            used += nbytes
+            stats["batches"] += 1
+            stats["max_batch"] = max(stats["max_batch"], len(batch))
+            stats["peak_bytes"] = max(stats["peak_bytes"], used)
+            stats["max_inflight"] = max(stats["max_inflight"], len(flying))
+            continue
+        if flying:
+            # with a slot free and more requests to come, poll the oldest
+            # batch rather than block on it, so that arrivals are submitted
+            # while it runs
+            if len(flying) < inflight and todo and not codec.ready(flying[0][0]):
+                sleep(min(poll, max(0.0, todo[0].arrival - now)))
+                continue
+            ticket, batch, nbytes = flying.popleft()
+            codec.wait(ticket)
+            end = clock() - t0
+            for r in batch:
+                r.done = end
+            used -= nbytes
+            continue
+        sleep(max(0.0, todo[0].arrival - now))
     return stats
 
 

@@ -132,6 +132,9 @@ SIGNATURES = {
     "zfp_hip_batch_count": (_SZ, [_VP]),
     "zfp_hip_batch_tiling": (_D, [_VP, P(_U), P(_U)]),
     "zfp_hip_batch_free": (None, [_VP]),
+    "zfp_hip_batch_submit": (_VP, [ctypes.c_int, P(_SP), P(_FP), _SZ, _U, _VP]),
+    "zfp_hip_batch_wait": (_SZ, [_VP, P(_SZ)]),
+    "zfp_hip_batch_ready": (_B, [_VP]),
 }
 BATCH_DEVICE_FIELDS = 1
 BATCH_NO_TUNE = 2
