@@ -269,9 +269,10 @@ def serving_leg(n_requests, rate, seed=2000):
     sizes log-uniform from 16 KiB to 256 MiB with the batched leg's mix of
     dimensionality, type and mode, fields and streams resident in HBM.  The
     same trace is replayed through the continuous-batching loop (admission by
-    an HBM budget of half the free memory) and one call per request; also
-    replayed with every request arriving at once (throughput when saturated).
-    Reports uncompressed GB/s and latency p50/p99."""
+    an HBM budget of half the free memory) with one batch at a time and with
+    two batches in flight on separate HIP streams, and one call per request;
+    also replayed with every request arriving at once (throughput when
+    saturated).  Reports uncompressed GB/s and latency p50/p99."""
     import math
 
     import numpy as np
@@ -334,23 +335,35 @@ def serving_leg(n_requests, rate, seed=2000):
     batched = serve.summary(trace)
     batched.update(batches=stats["batches"], max_batch=stats["max_batch"])
     note(f"batched: {batched}")
+    # two batches in flight on separate HIP streams
+    replay(lambda: serve.serve_batched(trace, codec, budget, inflight=2))  # warm the pool
+    ostats = replay(lambda: serve.serve_batched(trace, codec, budget, inflight=2))
+    overlapped = serve.summary(trace)
+    overlapped.update(batches=ostats["batches"], max_batch=ostats["max_batch"],
+                      submitted_while_another_ran=ostats["overlapped"])
+    note(f"overlapped: {overlapped}")
     replay(lambda: serve.serve_one_at_a_time(trace, codec))
     single = serve.summary(trace)
     note(f"one call per request: {single}")
     replay(lambda: serve.serve_batched(trace, codec, budget), at_once=True)
     sat_b = serve.summary(trace)
+    replay(lambda: serve.serve_batched(trace, codec, budget, inflight=2), at_once=True)
+    sat_o = serve.summary(trace)
     replay(lambda: serve.serve_one_at_a_time(trace, codec), at_once=True)
     sat_s = serve.summary(trace)
-    note(f"saturated: batched {sat_b['GBps']} GB/s, one call per request {sat_s['GBps']} GB/s")
+    note(f"saturated: batched {sat_b['GBps']} GB/s, two in flight {sat_o['GBps']} GB/s, "
+         f"one call per request {sat_s['GBps']} GB/s")
     for r in trace:
         r.stream.close()
     del trace
     torch.cuda.empty_cache()
     return {"rate_rps": rate, "requests": n_requests, "uncompressed_bytes": total,
-            "hbm_budget_bytes": budget, "batched": batched, "per_request": single,
-            "saturated": {"batched_GBps": sat_b["GBps"], "per_request_GBps": sat_s["GBps"],
-                          "speedup": round(sat_b["GBps"] / sat_s["GBps"], 2),
-                          "batched_p99_ms": sat_b["p99_ms"]}}
+            "hbm_budget_bytes": budget, "batched": batched, "overlapped": overlapped,
+            "per_request": single,
+            "saturated": {"batched_GBps": sat_b["GBps"], "overlapped_GBps": sat_o["GBps"],
+                          "per_request_GBps": sat_s["GBps"],
+                          "speedup": round(max(sat_b["GBps"], sat_o["GBps"]) / sat_s["GBps"], 2),
+                          "batched_p99_ms": sat_b["p99_ms"], "overlapped_p99_ms": sat_o["p99_ms"]}}
 
 
 def variable_rate_leg(steps=5):
