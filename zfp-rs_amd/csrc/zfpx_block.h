@@ -306,53 +306,64 @@ template <> struct PWord<64> { typedef uint64_t T; };
 
 // Planes are handled CH at a time (a chunk).  The decoder keeps a chunk of
 // decoded planes live beside the coefficients, and smaller chunks help it fit
-// four waves per SIMD; the encoder transposes fewer, larger chunks.
+// four waves per SIMD; the encoder transposes 16 planes at a time.
 constexpr int kEncodeChunk = 16;
 constexpr int kDecodeChunk = 8;
 
-// Planes CH*c .. CH*c+CH-1 of a block (CH = 8 or 16): out[i] bit j = bit
-// CH*c+i of coefficient j.  These are rows of the transpose of the 32 x 32 bit
-// matrix of each 32 coefficients (row r = coefficient r); the stages that
-// separate rows are only computed for the half or quarter needed.
-template <int P, int SIZE, int CH>
-ZX_HD void plane_chunk(const typename Ints<P>::U (&u)[SIZE], int c,
-                       typename PWord<SIZE>::T (&out)[CH])
+// Planes 16c .. 16c+15 of a block come from the transpose of the 32 x 32 bit
+// matrix of each 32 coefficients (row r = coefficient r), of which only the 16
+// rows needed are computed.  The first step packs, per group of 32
+// coefficients, the 16 bits of chunk c of coefficients i and i + 16 into one
+// word (i < 16): chunk_halves.  The encoder takes this step for every chunk
+// before coding any, so that it holds these packed halves rather than the
+// coefficients while it codes.
+template <int SIZE> struct Halves {
+  static constexpr int G = SIZE > 32 ? 2 : 1;
+  uint32_t o[G][16];
+};
+
+template <int P, int SIZE>
+ZX_HD void chunk_halves(const typename Ints<P>::U (&u)[SIZE], int c, Halves<SIZE> &hv)
 {
-  static_assert(CH == 8 || CH == 16, "chunk of 8 or 16 planes");
-  typedef typename PWord<SIZE>::T W;
-  constexpr int G = SIZE > 32 ? 2 : 1;
-  const int first = CH * c;
-  const int h = first >> 5;            // 32-bit half of the coefficients
+  const int first = 16 * c;
+  const int h = first >> 5;             // 32-bit half of the coefficients
   const bool hi16 = (first & 16) != 0;  // rows 16..31 of the half
-  const bool hi8 = (first & 8) != 0;    // rows 8..15 of those (CH == 8)
   ZX_UNROLL
-  for (int g = 0; g < G; g++) {
+  for (int g = 0; g < Halves<SIZE>::G; g++) {
     uint32_t a[32];
     ZX_UNROLL
     for (int r = 0; r < 32; r++) {
       const int i = 32 * g + r;
       a[r] = i < SIZE ? (uint32_t)((uint64_t)u[i < SIZE ? i : 0] >> (32 * h)) : 0u;
     }
-    uint32_t o[16];
     ZX_UNROLL
-    for (int i = 0; i < 16; i++)
-      o[i] = hi16 ? (a[i] >> 16) | (a[i + 16] & 0xffff0000u) : (a[i] & 0x0000ffffu) | (a[i + 16] << 16);
-    uint32_t q[CH];
-    if constexpr (CH == 16) {
-      tr_stage<8>(o);
-      ZX_UNROLL
-      for (int i = 0; i < 16; i++) q[i] = o[i];
-    } else {
-      ZX_UNROLL
-      for (int i = 0; i < 8; i++)
-        q[i] = hi8 ? ((o[i] >> 8) & 0x00ff00ffu) | (o[i + 8] & 0xff00ff00u)
-                   : (o[i] & 0x00ff00ffu) | ((o[i + 8] << 8) & 0xff00ff00u);
+    for (int i = 0; i < 16; i++) {
+      hv.o[g][i] = hi16 ? (a[i] >> 16) | (a[i + 16] & 0xffff0000u) : (a[i] & 0x0000ffffu) | (a[i + 16] << 16);
+#if defined(__HIP_DEVICE_COMPILE__)
+      // computed here, not sunk into the chunk's branch (where the
+      // coefficients would have to stay live until then)
+      asm volatile("" : "+v"(hv.o[g][i]));
+#endif
     }
+  }
+}
+
+// The rest of the transpose: out[i] bit j = bit 16c+i of coefficient j.
+template <int SIZE>
+ZX_HD void chunk_planes(const Halves<SIZE> &hv, typename PWord<SIZE>::T (&out)[16])
+{
+  typedef typename PWord<SIZE>::T W;
+  ZX_UNROLL
+  for (int g = 0; g < Halves<SIZE>::G; g++) {
+    uint32_t q[16];
+    ZX_UNROLL
+    for (int i = 0; i < 16; i++) q[i] = hv.o[g][i];
+    tr_stage<8>(q);
     tr_stage<4>(q);
     tr_stage<2>(q);
     tr_stage<1>(q);
     ZX_UNROLL
-    for (int i = 0; i < CH; i++) {
+    for (int i = 0; i < 16; i++) {
       if (g == 0) out[i] = (W)q[i];
       else out[i] |= (W)((uint64_t)q[i] << 32);
     }
@@ -836,12 +847,16 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   const uint32_t kstart = (uint32_t)P - empty;  // planes below kstart remain
   uint32_t n = 0;
   constexpr int CH = kEncodeChunk;
+  static_assert(CH == 16, "the encoder's chunks are 16 planes");
+  Halves<SIZE> hv[P / CH];
+  ZX_UNROLL
+  for (int c = 0; c < P / CH; c++) chunk_halves<P, SIZE>(u, c, hv[c]);
   ZX_UNROLL
   for (int c = P / CH - 1; c >= 0; c--) {
     const bool need = bits != 0 && kstart > (uint32_t)(CH * c) && kmin <= (uint32_t)(CH * c + CH - 1);
     if (!wave_any(need)) continue;
     PW ch[CH];
-    plane_chunk<P, SIZE, CH>(u, c, ch);
+    chunk_planes<SIZE>(hv[c], ch);
     ZX_UNROLL
     for (int i = CH - 1; i >= 0; i--) {
       const uint32_t k = (uint32_t)(CH * c + i);

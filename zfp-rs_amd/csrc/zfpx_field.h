@@ -71,6 +71,16 @@ ZX_HD void set_block_grid(FieldDesc &f)
 
 ZX_HD uint32_t pad_src(uint32_t i, uint32_t n) { return i < n ? i : (i == 3 ? 0u : n - 1); }
 
+// Between the rows of a scalar gather or scatter: keeps the compiler from
+// computing all 64 element addresses up front (128 VGPRs of 64-bit pointers),
+// which would set the register budget of every kernel that contains the path.
+ZX_HD void row_fence()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" ::: "memory");
+#endif
+}
+
 struct BlockPos {
   int64_t origin;      // element offset of the block origin from data
   uint32_t len[3];     // values inside the field along each axis
@@ -127,12 +137,20 @@ ZX_HD void gather(const FieldDesc &f, const BlockPos &p,
       v[i] = base[off];
     }
   } else {
+    // per-axis offsets of the padded positions, then one row of four at a
+    // time (see row_fence)
+    int64_t o[3][4];
     ZX_UNROLL
-    for (int i = 0; i < Blk<D>::SIZE; i++) {
-      int64_t off = (int64_t)pad_src((uint32_t)(i & 3), p.len[0]) * f.s[0];
-      if (D >= 2) off += (int64_t)pad_src((uint32_t)((i >> 2) & 3), p.len[1]) * f.s[1];
-      if (D >= 3) off += (int64_t)pad_src((uint32_t)((i >> 4) & 3), p.len[2]) * f.s[2];
-      v[i] = base[off];
+    for (int a = 0; a < 3; a++) {
+      ZX_UNROLL
+      for (int k = 0; k < 4; k++) o[a][k] = a < D ? (int64_t)pad_src((uint32_t)k, p.len[a]) * f.s[a] : 0;
+    }
+    ZX_UNROLL
+    for (int r = 0; r < Blk<D>::SIZE / 4; r++) {
+      const U *row = base + o[1][r & 3] + o[2][(r >> 2) & 3];
+      ZX_UNROLL
+      for (int x = 0; x < 4; x++) v[4 * r + x] = row[o[0][x]];
+      row_fence();
     }
   }
 }
@@ -145,15 +163,14 @@ ZX_HD void scatter(const FieldDesc &f, const BlockPos &p,
   typedef typename Ints<ST<S>::P>::U U;
   U *base = (U *)f.data + p.origin;
   ZX_UNROLL
-  for (int i = 0; i < Blk<D>::SIZE; i++) {
-    const uint32_t x = i & 3, y = (i >> 2) & 3, z = (i >> 4) & 3;
-    bool in = p.full || (x < p.len[0] && (D < 2 || y < p.len[1]) && (D < 3 || z < p.len[2]));
-    if (in) {
-      int64_t off = (int64_t)x * f.s[0];
-      if (D >= 2) off += (int64_t)y * f.s[1];
-      if (D >= 3) off += (int64_t)z * f.s[2];
-      base[off] = v[i];
-    }
+  for (int r = 0; r < Blk<D>::SIZE / 4; r++) {
+    const uint32_t y = r & 3, z = (r >> 2) & 3;
+    const bool row_in = p.full || ((D < 2 || y < p.len[1]) && (D < 3 || z < p.len[2]));
+    U *row = base + (D >= 2 ? (int64_t)y * f.s[1] : 0) + (D >= 3 ? (int64_t)z * f.s[2] : 0);
+    ZX_UNROLL
+    for (int x = 0; x < 4; x++)
+      if (row_in && (p.full || (uint32_t)x < p.len[0])) row[(int64_t)x * f.s[0]] = v[4 * r + x];
+    row_fence();
   }
 }
 

@@ -165,9 +165,10 @@ __global__ __launch_bounds__(64) void k_walk(DecodeArgs a, uint32_t *lengths)
 // ------------------------------------------------------------------ LDS path
 
 // Fixed-rate encode through LDS (zfpx_fx.h): one workgroup per group of
-// blockDim.x blocks.
+// blockDim.x blocks.  Held to four waves per SIMD (128 VGPRs; the coder keeps
+// packed chunk halves rather than the coefficients, so nothing spills).
 template <class S, int D, bool REV, bool OWN>
-__global__ __launch_bounds__(kThreads) void k_encode_fx(EncodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_encode_fx(EncodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   expand8_fill();
