@@ -164,11 +164,16 @@ __global__ __launch_bounds__(64) void k_walk(DecodeArgs a, uint32_t *lengths)
 
 // ------------------------------------------------------------------ LDS path
 
+// Waves per SIMD the LDS kernels are held to: four for lossy 32-bit values
+// (128 VGPRs); none for 64-bit values, whose 64 coefficients alone take 128,
+// nor for reversible blocks, which would spill.
+template <class S, bool REV> constexpr int kMinWaves = ST<S>::P == 32 && !REV ? 4 : 1;
+
 // Fixed-rate encode through LDS (zfpx_fx.h): one workgroup per group of
-// blockDim.x blocks.  Held to four waves per SIMD (128 VGPRs; the coder keeps
-// packed chunk halves rather than the coefficients, so nothing spills).
+// blockDim.x blocks.  32-bit values fit four waves per SIMD without spilling
+// (the coder keeps packed chunk halves rather than the coefficients).
 template <class S, int D, bool REV, bool OWN>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_encode_fx(EncodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<S, REV>, 8))) void k_encode_fx(EncodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   expand8_fill();
@@ -188,11 +193,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     if (!fx_enc_store<OWN>(a.words, a.nwords, lds, p, q, i, a.fixed)) break;
 }
 
-// Fixed-rate decode through LDS: stage the group's words, then decode.  Held
-// to four waves per SIMD (128 VGPRs, a few spilled): the group-test loop waits
-// on LDS reads, and the extra wave hides more of that than the spills cost.
+// Fixed-rate decode through LDS: stage the group's words, then decode.  For
+// 32-bit values held to four waves per SIMD (128 VGPRs, a few values spilled
+// once per block): the group-test loop waits on LDS reads, and the extra wave
+// hides more of that than the spills cost.
 template <class S, int D, bool REV, bool FLAT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<S, REV>, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
