@@ -24,7 +24,7 @@ def lib():
         L.hc_compress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                   P(ctypes.c_int64), P(ctypes.c_int64), P(HcCfg),
                                   P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
-                                  ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32]
+                                  ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32]
         L.hc_compress.restype = ctypes.c_uint64
         L.hc_divide.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]
         L.hc_divide.restype = None
@@ -47,8 +47,9 @@ def cfg_of(oc):
     return HcCfg(oc.minbits, oc.maxbits, oc.maxprec, oc.minexp, oc.rounding, oc.tight)
 
 
-def compress(arr, ztype, oc, words, start_bit, fixed_bits, rev, lds_T=0):
-    """lds_T > 0 runs the fixed-rate LDS path with workgroups of lds_T blocks."""
+def compress(arr, ztype, oc, words, start_bit, fixed_bits, rev, lds_T=0, lds_cap=0):
+    """lds_T > 0 runs the LDS path with workgroups of lds_T blocks; lds_cap > 0
+    sets the words a variable-rate workgroup may stage (else the default)."""
     n, s = _dims(arr)
     nb = 1
     for v in list(reversed(arr.shape)):
@@ -58,7 +59,7 @@ def compress(arr, ztype, oc, words, start_bit, fixed_bits, rev, lds_T=0):
     end = lib().hc_compress(ztype, arr.ndim, int(rev), arr.ctypes.data, n, s, ctypes.byref(c),
                             words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
                             start_bit, fixed_bits,
-                            lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), lds_T)
+                            lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), lds_T, lds_cap)
     return end, lens
 
 

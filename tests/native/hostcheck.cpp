@@ -121,11 +121,65 @@ static uint64_t decompress_fx(const FieldDesc &f, const Cfg &c, const uint64_t *
   return maxword;
 }
 
+// The variable-rate LDS path: the length pass, the scan, then the write pass
+// one workgroup at a time, staged through LDS or, for a workgroup over the
+// capacity, block by block (the kernels' second launch; its blocks' bits are
+// disjoint from the staged workgroups', so the order does not matter).
+template <class S, int D, bool REV>
+static uint64_t compress_vx(const FieldDesc &f, const Cfg &c, uint64_t *words, uint64_t nwords,
+                            uint64_t start, uint32_t *lens, uint32_t T, uint32_t cap)
+{
+  typedef typename Ints<ST<S>::P>::U U;
+  std::vector<uint64_t> offs(f.nblocks + 1);
+  offs[0] = start;
+  for (uint64_t b = 0; b < f.nblocks; b++) {
+    CountSink cs{0};
+    encode_field_block<S, D, REV>(f, b, cs, c);
+    lens[b] = (uint32_t)cs.n;
+    offs[b + 1] = offs[b] + lens[b];
+  }
+  FxPlan p;
+  vx_plan(offs[f.nblocks] - start, f.nblocks, T, cap, p);
+  std::vector<uint64_t> lds(p.lds_words);
+  const uint64_t groups = (f.nblocks + p.T - 1) / p.T;
+  for (uint64_t g = 0; g < groups; g++) {
+    const FxGroup q = vx_group(g, p.T, f.nblocks, offs.data());
+    const bool staged = q.nw <= p.cap;
+    for (auto &x : lds) x = 0x5555aaaa5555aaaaull;  // LDS starts undefined
+    if (staged)
+      for (uint32_t i = 0; i < q.nw; i++) fx_enc_clear(lds.data(), p, i);
+    for (uint32_t t = 0; t < q.nb; t++) {
+      U v[Blk<D>::SIZE];
+      gather<S, D>(f, block_pos<D>(f, q.b0 + t), v);
+      const uint64_t pos = offs[q.b0 + t];
+      if (staged) {
+        fx_enc_at<S, D, REV>(lds.data(), p, (uint32_t)(pos - 64 * q.w0), v, c);
+      } else {
+        WordSink w;
+        w.init(words, nwords, pos);
+        encode_values<S, D, REV>(w, v, c);
+        w.finish();
+      }
+    }
+    if (staged)
+      for (uint32_t i = 0; i < q.nw; i++)
+        if (!fx_enc_store<false>(words, nwords, lds.data(), p, q, i, 0)) break;
+  }
+  return offs[f.nblocks];
+}
+
 template <class S, bool REV>
 static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, uint64_t nw,
-                       uint64_t st, uint32_t fx, uint32_t *l, uint32_t T)
+                       uint64_t st, uint32_t fx, uint32_t *l, uint32_t T, uint32_t cap)
 {
   FxPlan p;
+  if (T && !fx) {
+    switch (dims) {
+    case 1: return compress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);
+    case 2: return compress_vx<S, 2, REV>(f, c, w, nw, st, l, T, cap);
+    default: return compress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
+    }
+  }
   if (T && fx_plan(fx, st, false, T, p)) {
     switch (dims) {
     case 1: return compress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
@@ -164,19 +218,20 @@ extern "C" {
 
 uint64_t hc_compress(int type, int dims, int rev, void *data, const int64_t *n, const int64_t *s,
                      const HcCfg *hc, uint64_t *words, uint64_t nwords, uint64_t start_bit,
-                     uint32_t fixed_bits, uint32_t *lengths, uint32_t lds_T)
+                     uint32_t fixed_bits, uint32_t *lengths, uint32_t lds_T,
+                     uint32_t lds_cap)
 {
   FieldDesc f = make_field(dims, data, n, s);
   Cfg c{hc->minbits, hc->maxbits, hc->maxprec, hc->minexp, hc->rounding, hc->tight};
   switch (type * 2 + (rev ? 1 : 0)) {
-  case 2: return comp_d<int32_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
-  case 3: return comp_d<int32_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
-  case 4: return comp_d<int64_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
-  case 5: return comp_d<int64_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
-  case 6: return comp_d<float, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
-  case 7: return comp_d<float, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
-  case 8: return comp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
-  default: return comp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
+  case 2: return comp_d<int32_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
+  case 3: return comp_d<int32_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
+  case 4: return comp_d<int64_t, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
+  case 5: return comp_d<int64_t, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
+  case 6: return comp_d<float, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
+  case 7: return comp_d<float, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
+  case 8: return comp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
+  default: return comp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T, lds_cap);
   }
 }
 

@@ -53,6 +53,15 @@ def data_for(ztype, shape, kind, seed):
     return specials(shape, dt, seed)
 
 
+def median_group_words(lens, start_bits, T):
+    """The median count of stream words that workgroups of T blocks touch."""
+    offs = start_bits + np.concatenate([[0], np.cumsum(lens.astype(np.int64))])
+    b0 = np.arange(0, lens.size, T)
+    b1 = np.minimum(b0 + T, lens.size)
+    nw = (offs[b1] + 63) // 64 - offs[b0] // 64
+    return max(int(np.median(nw)), 1)
+
+
 def run_case(arr, ztype, cfg, start_bits):
     dims = arr.ndim
     n, err, ow = O.compress(arr, cfg, start_bits=start_bits)
@@ -64,23 +73,34 @@ def run_case(arr, ztype, cfg, start_bits):
     assert rerr == 0
     isz = arr.dtype.itemsize
     u = np.uint32 if isz == 4 else np.uint64
-    # the global-memory path, and for fixed-rate streams the LDS path with
-    # workgroups of 3 blocks (group boundaries inside the field) and of 64
-    for lds_T in ((0, 3, 64) if fixed and fixed <= 4096 else (0,)):
+    # the global-memory path, and the LDS path with workgroups of 3 blocks
+    # (group boundaries inside the field) and of 64; variable-rate streams (whose
+    # LDS path is the write pass; they decode from global memory) also with a
+    # staging capacity of the median workgroup's words, so that dense
+    # workgroups are written block by block beside staged ones
+    if fixed:
+        variants = [(0, 0), (3, 0), (64, 0)] if fixed <= 4096 else [(0, 0)]
+    else:
+        variants = [(0, 0), (3, 0), (64, 0), (3, None)]
+    for lds_T, lds_cap in variants:
+        if lds_cap is None:
+            lds_cap = median_group_words(lens0, start_bits, lds_T)
         words = np.zeros(nwords + 4, dtype=np.uint64)
         words[: start_bits // 64] = PREFIX_WORD
         if start_bits % 64:
             words[start_bits // 64] = PREFIX_PART & ((1 << (start_bits % 64)) - 1)
-        end, lens = H.compress(arr, ztype, cfg, words, start_bits, fixed, rev, lds_T)
-        assert (end + 63) // 64 * 8 == n, f"stream length (lds_T={lds_T})"
-        assert np.array_equal(words[:nwords], ow[:nwords]), f"stream words (lds_T={lds_T})"
+        end, lens = H.compress(arr, ztype, cfg, words, start_bits, fixed, rev, lds_T, lds_cap)
+        assert (end + 63) // 64 * 8 == n, f"stream length (lds_T={lds_T}, cap={lds_cap})"
+        assert np.array_equal(words[:nwords], ow[:nwords]), f"stream words (lds_T={lds_T}, cap={lds_cap})"
         assert not words[nwords:].any()
         got, loaded = H.decompress(arr.shape, arr.dtype, ztype, cfg, words, start_bits, fixed,
                                    None if fixed else lens, rev, lds_T)
-        assert np.array_equal(got.view(u), ref.view(u)), f"decoded values (lds_T={lds_T})"
-        if lds_T:
+        assert np.array_equal(got.view(u), ref.view(u)), f"decoded values (lds_T={lds_T}, cap={lds_cap})"
+        if lds_T == 0:
+            lens0 = lens
+        if lds_T and fixed:
             # a whole stream is read without loading past its end
-            assert 0 < loaded <= nwords, f"loaded words (lds_T={lds_T})"
+            assert 0 < loaded <= nwords, f"loaded words (lds_T={lds_T}, cap={lds_cap})"
 
 
 def test_lds_path_capacity_and_overread():

@@ -206,6 +206,31 @@ def test_variable_rate_index_and_walk():
         x.close()
 
 
+@pytest.mark.parametrize("ztype", TYPES)
+def test_variable_rate_sparse_and_dense_workgroups(ztype):
+    """A field that is constant but for a noisy patch: the workgroups over the
+    constant part stage their words through LDS, those over the patch hold far
+    more than the mean and are coded block by block from HBM; both give the
+    oracle's bytes and values."""
+    dt = O.DTYPES[ztype]
+    rng = np.random.default_rng(40 + ztype)
+    arr = np.zeros((64, 64, 64), dtype=dt)
+    patch = (slice(18, 30), slice(0, 64), slice(0, 40))
+    shape = arr[patch].shape
+    if np.dtype(dt).kind == "f":
+        arr[...] = 1.5
+        arr[patch] = rng.standard_normal(shape).astype(dt) * 1e3
+        cases = [("precision 28", O.fixed_precision(28)), ("accuracy 1e-3", O.fixed_accuracy(1e-3))]
+    else:
+        arr[...] = 7
+        arr[patch] = rng.integers(-2 ** 20, 2 ** 20, size=shape).astype(dt)
+        cases = [("precision 24", O.fixed_precision(24))]
+    cases.append(("reversible", O.reversible()))
+    for name, oc in cases:
+        for device in (False, True):
+            check(arr, oc, 0, device=device, ctx=f"{name} device={device}")
+
+
 def test_buffer_too_small_matches_reference_words():
     """An undersized buffer: compress reports failure, and the words that fit are the
     reference's (writes past the end are dropped, core.rs:159-170)."""

@@ -819,6 +819,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
     end_rel = *(uint64_t *)W.one;
     a.lengths = nullptr;
     a.offsets = offs;
+    a.vbits = end_rel - a.start;
   }
   const uint64_t w1 = (64 * w0 + end_rel + 63) / 64;  // words through the flush
   const uint64_t cap_words = s->nwords > w0 ? s->nwords - w0 : 0;

@@ -20,6 +20,10 @@ struct EncodeArgs {
   uint32_t *lengths;        // non-null: length pass, write bit counts here
   bool vec;                 // rows of whole blocks may be moved as 16-byte vectors
   uint32_t slice;           // blocks per workgroup of the LDS path, or 0 for its default
+  uint64_t vbits;           // variable placement: bits of all blocks (offsets[nblocks] -
+                            // offsets[0]), sizing the LDS path; 0 keeps to global memory
+  uint32_t vcap;            // variable placement: words a workgroup may stage, or 0 for
+                            // the default (from vbits)
 };
 
 struct DecodeArgs {

@@ -38,6 +38,7 @@ struct FxPlan {
                        // 2^skew is a multiple of their word count, so no pad slot
                        // splits a block (readers address their words directly)
   uint32_t lds_words;  // LDS words per workgroup
+  uint32_t cap;        // variable-rate: the most words a staged group may touch
 };
 
 ZX_HD uint32_t fx_log2(uint32_t v)
@@ -62,6 +63,7 @@ ZX_HD bool fx_plan(uint32_t fixed, uint64_t start, bool decode, uint32_t T, FxPl
   const uint32_t s = fx_log2(W);
   p.skew = p.own ? 31u : s < 3 ? 3u : s;
   p.flat = decode && fixed % 64 == 0 && start % 64 == 0 && ((1u << p.skew) % W) == 0;
+  p.cap = 0;
   if (p.own) {
     p.lds_words = p.T * (W + 1);
   } else {
@@ -98,6 +100,47 @@ ZX_HD FxGroup fx_group(uint64_t g, uint32_t T, uint64_t nblocks, uint64_t start,
 
 ZX_HD uint32_t fx_slot(uint32_t i, uint32_t skew) { return i + (i >> skew); }
 
+// The variable-rate write pass takes the same encode phases, with each block
+// placed by the scan of the block lengths (offsets[b] is block b's first bit
+// and offsets[nblocks] the end) in the shared layout with a pad word every 8.
+// A workgroup stages its range when it touches at most `cap` words; a denser
+// range is written block by block to HBM.  The LDS a launch reserves bounds
+// how many workgroups fit on a CU, so the capacity follows the stream's mean
+// block length: room for 1.5 mean groups, at most 64 KiB.  `T` (at most 256)
+// overrides the workgroup size and `cap` the capacity when nonzero.  (Decoding
+// variable-rate streams through LDS measured slower than reading HBM block by
+// block, so decode keeps the global-memory path.)
+ZX_HD void vx_plan(uint64_t bits, uint64_t nblocks, uint32_t T, uint32_t cap, FxPlan &p)
+{
+  const uint64_t mean = nblocks ? (bits + nblocks - 1) / nblocks : 0;
+  const uint64_t W = (mean + 63) / 64;
+  p.T = T ? T : W <= 16 ? 256u : W <= 32 ? 128u : 64u;
+  p.skew = 3;
+  p.own = false;
+  p.flat = false;
+  // words 0 .. cap + 1 in slots
+  const uint64_t most = (kFxMaxLdsBytes / 8 - 1) * 8 / 9 - 2;
+  const uint64_t want = cap ? cap : (mean * p.T * 3 / 2 + 63) / 64 + 16;
+  p.cap = (uint32_t)(want < most ? want : most);
+  p.lds_words = fx_slot(p.cap + 1, p.skew) + 1;
+}
+
+ZX_HD FxGroup vx_group(uint64_t g, uint32_t T, uint64_t nblocks, const uint64_t *offsets)
+{
+  FxGroup q;
+  q.b0 = g * T;
+  q.nb = nblocks - q.b0 < T ? (uint32_t)(nblocks - q.b0) : T;
+  const uint64_t bit0 = offsets[q.b0];
+  const uint64_t bit_end = offsets[q.b0 + q.nb];
+  q.w0 = bit0 >> 6;
+  q.r0 = (uint32_t)(bit0 & 63);
+  const uint64_t nw = ((bit_end + 63) >> 6) - q.w0;
+  q.nw = nw > 0xffffffffull ? 0xffffffffu : (uint32_t)nw;
+  q.first_shared = q.r0 != 0;
+  q.last_shared = (bit_end & 63) != 0;
+  return q;
+}
+
 // Words a block loaded, as the reference's cursor counts them (one past the
 // highest): those it consumed bits of, plus the word a skip to min_bits lands
 // inside (decompress.rs:57-74, bitstream/core.rs:265-284).  Positions are
@@ -119,6 +162,18 @@ ZX_HD uint64_t loaded_words(const Src &r, uint64_t pos, uint32_t minbits)
 // phase 1 (shared layout only): zero word slot i
 ZX_HD void fx_enc_clear(uint64_t *lds, const FxPlan &p, uint32_t i) { lds[fx_slot(i, p.skew)] = 0; }
 
+// phase 2 (shared layout): a block's values v encoded from bit `rel` of the
+// group's first word
+template <class S, int D, bool REV>
+ZX_HD void fx_enc_at(uint64_t *lds, const FxPlan &p, uint32_t rel,
+                     const typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE], const Cfg &c)
+{
+  LdsSink<true> w;
+  w.init(lds, p.skew, rel);
+  encode_values<S, D, REV>(w, v, c);
+  w.finish();
+}
+
 // phase 2: thread t encodes its block's values v
 template <class S, int D, bool REV, bool OWN>
 ZX_HD void fx_enc_block(uint64_t *lds, const FxPlan &p, const FxGroup &q, uint32_t t, uint32_t fixed,
@@ -130,10 +185,7 @@ ZX_HD void fx_enc_block(uint64_t *lds, const FxPlan &p, const FxGroup &q, uint32
     encode_values<S, D, REV>(w, v, c);
     w.finish();
   } else {
-    LdsSink<true> w;
-    w.init(lds, p.skew, q.r0 + t * fixed);
-    encode_values<S, D, REV>(w, v, c);
-    w.finish();
+    fx_enc_at<S, D, REV>(lds, p, q.r0 + t * fixed, v, c);
   }
 }
 

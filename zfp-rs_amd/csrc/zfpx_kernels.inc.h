@@ -215,6 +215,64 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
   store_block<S, D>(a.field, b, a.vec, v);
 }
 
+// Variable-rate write pass through LDS: a workgroup whose blocks touch at most
+// p.cap words encodes them into LDS at their scanned offsets and stores the
+// range with coalesced stores (atomic ORs for the two words it may share with
+// its neighbours).  Denser workgroups are left to k_encode_dense.  The choice
+// is uniform over the workgroup, so the barriers are too.
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<S, REV>, 8))) void k_encode_vx(EncodeArgs a, FxPlan p)
+{
+  extern __shared__ uint64_t lds[];
+  const uint32_t t = threadIdx.x;
+  const FxGroup q = vx_group(blockIdx.x, p.T, a.field.nblocks, a.offsets);
+  if (q.nw > p.cap) return;
+  expand8_fill();
+  for (uint32_t i = t; i < q.nw; i += p.T) fx_enc_clear(lds, p, i);
+  __syncthreads();
+  if (t < q.nb) {
+    typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+    load_block<S, D>(a.field, q.b0 + t, a.vec, v);
+    fx_enc_at<S, D, REV>(lds, p, (uint32_t)(a.offsets[q.b0 + t] - 64 * q.w0), v, a.cfg);
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < q.nw; i += p.T)
+    if (!fx_enc_store<false>(a.words, a.nwords, lds, p, q, i, 0)) break;
+}
+
+// The groups k_encode_vx leaves: each block written to HBM at its offset, as
+// k_encode does.  Shared words are merged with atomic ORs, so the two launches
+// may run in either order.  Most streams have few or no dense groups, so a
+// workgroup first tests blockDim.x groups at once (one per thread) and then
+// codes the dense ones among them.
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(kThreads) void k_encode_dense(EncodeArgs a, FxPlan p, uint64_t ngroups)
+{
+  __shared__ uint32_t dense[kThreads];
+  __shared__ uint32_t ndense;
+  const uint32_t t = threadIdx.x;
+  if (t == 0) ndense = 0;
+  __syncthreads();
+  const uint64_t g0 = (uint64_t)blockIdx.x * p.T;
+  if (g0 + t < ngroups && vx_group(g0 + t, p.T, a.field.nblocks, a.offsets).nw > p.cap)
+    dense[atomicAdd(&ndense, 1u)] = t;
+  __syncthreads();
+  const uint32_t n = ndense;
+  if (n == 0) return;
+  expand8_fill();
+  for (uint32_t k = 0; k < n; k++) {
+    const FxGroup q = vx_group(g0 + dense[k], p.T, a.field.nblocks, a.offsets);
+    if (t < q.nb) {
+      typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+      load_block<S, D>(a.field, q.b0 + t, a.vec, v);
+      WordSink w;
+      w.init(a.words, a.nwords, a.offsets[q.b0 + t]);
+      encode_values<S, D, REV>(w, v, a.cfg);
+      w.finish();
+    }
+  }
+}
+
 // ------------------------------------------------------------------ batches
 
 // The array holding a launch's block g: the last j with firsts[j] <= g.
@@ -309,6 +367,13 @@ static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
     const size_t bytes = (size_t)p.lds_words * 8;
     if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
     else hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
+    return hipGetLastError();
+  }
+  if (!a.lengths && a.offsets && a.vbits) {
+    vx_plan(a.vbits, a.field.nblocks, a.slice, a.vcap, p);
+    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+    hipLaunchKernelGGL((k_encode_vx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+    hipLaunchKernelGGL((k_encode_dense<S, D, REV>), dim3((grid + p.T - 1) / p.T), dim3(p.T), 0, s, a, p, grid);
     return hipGetLastError();
   }
   const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
