@@ -1153,36 +1153,31 @@ template <> struct Bits<int64_t> {
 };
 
 // fwd_cast of a whole block (encode/core.rs:323-423): q = trunc(x * 2^(P-2-emax)),
-// scaled in two exact steps (x * 2^64 first) when 2^(P-2-emax) is not a
-// normal power of two (emax < MINCAST), and the minimum integer wherever the
-// scaled value is not within +-2^(P-1) (NaN, infinity, overflow), as x86's
-// conversion gives.  The products are exact wherever they are at least one in
-// magnitude (both factors are powers of two and the first step cannot leave
-// the normal range), and smaller ones truncate to zero whatever their
-// rounding, so the hardware multiply and truncate give the exact result.
+// and the minimum integer wherever the scaled value is not within +-2^(P-1)
+// (NaN, infinity, overflow), as x86's conversion gives.  The scaling is one
+// ldexp (v_ldexp on the device), which is exact wherever the result is at
+// least one in magnitude, whatever emax (subnormal inputs included); smaller
+// results truncate to zero whatever their rounding, so ldexp and truncation
+// give the exact result.
 template <class S, int SIZE>
 ZX_HD void cast_block(const typename Ints<ST<S>::P>::U (&b)[SIZE], int emax,
                       typename Ints<ST<S>::P>::I (&q)[SIZE])
 {
   typedef typename Ints<ST<S>::P>::I I;
   constexpr int P = ST<S>::P;
-  const bool tiny = emax < ST<S>::MINCAST;
   const I imin = (I)((typename Ints<P>::U)1 << (P - 1));
+  const int k = P - 2 - emax;
   if constexpr (P == 32) {
-    const float s1 = tiny ? 0x1p64f : 1.0f;
-    const float s2 = pow2f(tiny ? 30 - 64 - emax : 30 - emax);
     ZX_UNROLL
     for (int i = 0; i < SIZE; i++) {
-      const float v = (u2f((uint32_t)b[i]) * s1) * s2;
+      const float v = ldexpf(u2f((uint32_t)b[i]), k);
       const bool in = fabsf(v) < 0x1p31f;
       q[i] = in ? (I)(in ? v : 0.0f) : imin;
     }
   } else {
-    const double s1 = tiny ? 0x1p64 : 1.0;
-    const double s2 = pow2d(tiny ? 62 - 64 - emax : 62 - emax);
     ZX_UNROLL
     for (int i = 0; i < SIZE; i++) {
-      const double v = (u2d((uint64_t)b[i]) * s1) * s2;
+      const double v = ldexp(u2d((uint64_t)b[i]), k);
       const bool in = fabs(v) < 0x1p63;
       q[i] = in ? (I)(in ? v : 0.0) : imin;
     }
