@@ -24,6 +24,7 @@
 
 #include "../../include/zfp.h"
 #include "../../include/zfp_hip.h"
+#include "../../include/zfp_block.h"
 #include "zfpx_api.h"
 #include "zfpx_fx.h"
 
@@ -774,9 +775,11 @@ bool stage_field(Workspace &W, const zfp_field *f, const Plan &p, bool copy_in, 
 
 // Encode every block of the field at the cursor of `s`; leaves the cursor
 // where compress + finish would (compress.rs:22-53).  Returns false on a
-// device failure (nothing consistent was written).
+// device failure (nothing consistent was written).  `end_bit` receives the
+// bit after the last block; `keep_index` keeps a variable-rate stream's
+// offset index beside it.
 bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st,
-                  ExecParams *xp)
+                  ExecParams *xp, bool keep_index = true, uint64_t *end_bit = nullptr)
 {
   Workspace &W = ws();
   std::lock_guard<std::mutex> lock(W.mu);
@@ -841,7 +844,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   if (!s->device && live) {
     ZCHECK(hipMemcpyAsync(s->words + w0, dwords, live * 8, hipMemcpyDeviceToHost, st));
   }
-  if (!fixed) {
+  if (!fixed && keep_index) {
     // keep the offset index beside the stream
     if (s->index_cap < p.nblocks) {
       drop_index(s);
@@ -862,6 +865,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   tm.mark(3);
   ZCHECK(hipStreamSynchronize(st));
   tm.collect();
+  if (end_bit) *end_bit = 64 * w0 + end_rel;
   // the cursor after the last block, then flush (core.rs:536-542)
   if (w1 > s->nwords) s->overflowed = true;
   s->word_pos = (size_t)w1;
@@ -973,6 +977,99 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   s->seek_word = s->word_pos;
   s->overread = over;
   return true;
+}
+
+// ---------------------------------------------------------------- single blocks
+
+// The plan of one block of `dims` axes with `len` values (1..4) and `strides`
+// per axis, taken as given (a zero stride repeats a value, as the reference's
+// pointer walk does); false for a length outside 1..4 (check_lengths,
+// src/codec/block/strided.rs:392-405) or a span that overflows.
+bool block_plan(zfp_type t, int dims, const size_t len[4], const ptrdiff_t str[4], Plan &p)
+{
+  if (!valid_type(t) || dims < 1 || dims > 4) return false;
+  p.type = t;
+  p.esz = type_size(t);
+  p.dims = dims;
+  p.imin = p.imax = 0;
+  p.nblocks = 1;
+  for (int a = 0; a < 4; a++) {
+    p.n[a] = 1;
+    p.s[a] = 0;
+  }
+  for (int a = 0; a < dims; a++) {
+    if (len[a] < 1 || len[a] > 4) return false;
+    const __int128 ext = (__int128)str[a] * (int64_t)(len[a] - 1);
+    if (ext > INT64_MAX / 4 || ext < INT64_MIN / 4) return false;
+    p.n[a] = (int64_t)len[a];
+    p.s[a] = str[a];
+    if (ext < 0) p.imin += (int64_t)ext;
+    else p.imax += (int64_t)ext;
+  }
+  for (int a = 0; a < 3; a++) p.nb[a] = 1;
+  return true;
+}
+
+// One block coded on the GPU at the stream's cursor, as the block-level
+// entry points of zfp-rs-ffi do (zfp-rs-ffi/src/block.rs, over
+// src/codec/block/strided.rs:420-700): the same coder as a one-block field,
+// with the cursor left right after the block (the writer holding the block's
+// last partial word, unflushed; the reader unaligned).  Returns the block's
+// bits, or 0 for a null block, a length outside 1..4, a 4-D block (not built
+// for the GPU) or a device failure.
+size_t block_code(bool encode, zfp_stream *z, const void *block, zfp_type t, int dims,
+                  const size_t len[4], const ptrdiff_t str[4])
+{
+  if (!z || !z->stream || !block) return 0;
+  Plan p;
+  if (!block_plan(t, dims, len, str, p) || dims > 3) return 0;
+  if (((uintptr_t)((const char *)block + p.imin * (int64_t)p.esz)) % p.esz) return 0;
+  zfp_field f{};
+  f.type = t;
+  f.data = const_cast<void *>(block);
+  bitstream *s = z->stream;
+  const zx::Cfg c = cfg_of(z);
+  uint64_t end = 0;
+  if (encode) {
+    const uint64_t start = s->wtell();
+    const bool was_over = s->overflowed;
+    if (!gpu_compress(s, &f, p, c, hip_stream_of(z), params(z, false), false, &end)) return 0;
+    // a whole word past the capacity was dropped
+    const bool over = was_over || end / 64 > s->nwords;
+    s->wseek(end);
+    s->overflowed = over;
+    return (size_t)(end - start);
+  }
+  const uint64_t start = s->rtell();
+  if (!gpu_decompress(s, &f, p, c, hip_stream_of(z), params(z, false), end)) return 0;
+  const bool over = s->overread;
+  s->rseek(end);
+  s->overread = over;
+  return (size_t)(end - start);
+}
+
+size_t block_whole(bool encode, zfp_stream *z, const void *b, zfp_type t, int dims)
+{
+  const size_t len[4] = {4, 4, 4, 4};
+  const ptrdiff_t str[4] = {1, 4, 16, 64};
+  return block_code(encode, z, b, t, dims, len, str);
+}
+
+size_t block_strided(bool encode, zfp_stream *z, const void *b, zfp_type t, int dims, ptrdiff_t sx,
+                     ptrdiff_t sy, ptrdiff_t sz, ptrdiff_t sw)
+{
+  const size_t len[4] = {4, 4, 4, 4};
+  const ptrdiff_t str[4] = {sx, sy, sz, sw};
+  return block_code(encode, z, b, t, dims, len, str);
+}
+
+size_t block_partial(bool encode, zfp_stream *z, const void *b, zfp_type t, int dims, size_t nx,
+                     size_t ny, size_t nz, size_t nw, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz,
+                     ptrdiff_t sw)
+{
+  const size_t len[4] = {nx, ny, nz, nw};
+  const ptrdiff_t str[4] = {sx, sy, sz, sw};
+  return block_code(encode, z, b, t, dims, len, str);
 }
 
 }  // namespace
@@ -1893,6 +1990,99 @@ size_t zfp_decompress(zfp_stream *z, zfp_field *f)
   if (s->overread_now()) return 0;
   return s->word_pos * 8;
 }
+
+// Block-level entry points (zfp-rs-ffi/src/block.rs): T the C scalar type,
+// N its name in the symbols, ZT its zfp_type.
+#define ZX_BLOCK_API(T, N, ZT)                                                                     \
+  size_t zfp_encode_block_##N##_1(zfp_stream *z, const T *b) { return block_whole(true, z, b, ZT, 1); } \
+  size_t zfp_encode_block_##N##_2(zfp_stream *z, const T *b) { return block_whole(true, z, b, ZT, 2); } \
+  size_t zfp_encode_block_##N##_3(zfp_stream *z, const T *b) { return block_whole(true, z, b, ZT, 3); } \
+  size_t zfp_encode_block_##N##_4(zfp_stream *z, const T *b) { return block_whole(true, z, b, ZT, 4); } \
+  size_t zfp_decode_block_##N##_1(zfp_stream *z, T *b) { return block_whole(false, z, b, ZT, 1); }      \
+  size_t zfp_decode_block_##N##_2(zfp_stream *z, T *b) { return block_whole(false, z, b, ZT, 2); }      \
+  size_t zfp_decode_block_##N##_3(zfp_stream *z, T *b) { return block_whole(false, z, b, ZT, 3); }      \
+  size_t zfp_decode_block_##N##_4(zfp_stream *z, T *b) { return block_whole(false, z, b, ZT, 4); }      \
+  size_t zfp_encode_block_strided_##N##_1(zfp_stream *z, const T *b, ptrdiff_t sx)                      \
+  {                                                                                                    \
+    return block_strided(true, z, b, ZT, 1, sx, 0, 0, 0);                                              \
+  }                                                                                                    \
+  size_t zfp_encode_block_strided_##N##_2(zfp_stream *z, const T *b, ptrdiff_t sx, ptrdiff_t sy)        \
+  {                                                                                                    \
+    return block_strided(true, z, b, ZT, 2, sx, sy, 0, 0);                                             \
+  }                                                                                                    \
+  size_t zfp_encode_block_strided_##N##_3(zfp_stream *z, const T *b, ptrdiff_t sx, ptrdiff_t sy,        \
+                                          ptrdiff_t sz)                                                \
+  {                                                                                                    \
+    return block_strided(true, z, b, ZT, 3, sx, sy, sz, 0);                                            \
+  }                                                                                                    \
+  size_t zfp_encode_block_strided_##N##_4(zfp_stream *z, const T *b, ptrdiff_t sx, ptrdiff_t sy,        \
+                                          ptrdiff_t sz, ptrdiff_t sw)                                  \
+  {                                                                                                    \
+    return block_strided(true, z, b, ZT, 4, sx, sy, sz, sw);                                           \
+  }                                                                                                    \
+  size_t zfp_decode_block_strided_##N##_1(zfp_stream *z, T *b, ptrdiff_t sx)                            \
+  {                                                                                                    \
+    return block_strided(false, z, b, ZT, 1, sx, 0, 0, 0);                                             \
+  }                                                                                                    \
+  size_t zfp_decode_block_strided_##N##_2(zfp_stream *z, T *b, ptrdiff_t sx, ptrdiff_t sy)              \
+  {                                                                                                    \
+    return block_strided(false, z, b, ZT, 2, sx, sy, 0, 0);                                            \
+  }                                                                                                    \
+  size_t zfp_decode_block_strided_##N##_3(zfp_stream *z, T *b, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz) \
+  {                                                                                                    \
+    return block_strided(false, z, b, ZT, 3, sx, sy, sz, 0);                                           \
+  }                                                                                                    \
+  size_t zfp_decode_block_strided_##N##_4(zfp_stream *z, T *b, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz, \
+                                          ptrdiff_t sw)                                                \
+  {                                                                                                    \
+    return block_strided(false, z, b, ZT, 4, sx, sy, sz, sw);                                          \
+  }                                                                                                    \
+  size_t zfp_encode_partial_block_strided_##N##_1(zfp_stream *z, const T *b, size_t nx, ptrdiff_t sx)   \
+  {                                                                                                    \
+    return block_partial(true, z, b, ZT, 1, nx, 1, 1, 1, sx, 0, 0, 0);                                 \
+  }                                                                                                    \
+  size_t zfp_encode_partial_block_strided_##N##_2(zfp_stream *z, const T *b, size_t nx, size_t ny,      \
+                                                  ptrdiff_t sx, ptrdiff_t sy)                          \
+  {                                                                                                    \
+    return block_partial(true, z, b, ZT, 2, nx, ny, 1, 1, sx, sy, 0, 0);                               \
+  }                                                                                                    \
+  size_t zfp_encode_partial_block_strided_##N##_3(zfp_stream *z, const T *b, size_t nx, size_t ny,      \
+                                                  size_t nz, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz) \
+  {                                                                                                    \
+    return block_partial(true, z, b, ZT, 3, nx, ny, nz, 1, sx, sy, sz, 0);                             \
+  }                                                                                                    \
+  size_t zfp_encode_partial_block_strided_##N##_4(zfp_stream *z, const T *b, size_t nx, size_t ny,      \
+                                                  size_t nz, size_t nw, ptrdiff_t sx, ptrdiff_t sy,    \
+                                                  ptrdiff_t sz, ptrdiff_t sw)                          \
+  {                                                                                                    \
+    return block_partial(true, z, b, ZT, 4, nx, ny, nz, nw, sx, sy, sz, sw);                           \
+  }                                                                                                    \
+  size_t zfp_decode_partial_block_strided_##N##_1(zfp_stream *z, T *b, size_t nx, ptrdiff_t sx)         \
+  {                                                                                                    \
+    return block_partial(false, z, b, ZT, 1, nx, 1, 1, 1, sx, 0, 0, 0);                                \
+  }                                                                                                    \
+  size_t zfp_decode_partial_block_strided_##N##_2(zfp_stream *z, T *b, size_t nx, size_t ny,            \
+                                                  ptrdiff_t sx, ptrdiff_t sy)                          \
+  {                                                                                                    \
+    return block_partial(false, z, b, ZT, 2, nx, ny, 1, 1, sx, sy, 0, 0);                              \
+  }                                                                                                    \
+  size_t zfp_decode_partial_block_strided_##N##_3(zfp_stream *z, T *b, size_t nx, size_t ny, size_t nz, \
+                                                  ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz)            \
+  {                                                                                                    \
+    return block_partial(false, z, b, ZT, 3, nx, ny, nz, 1, sx, sy, sz, 0);                            \
+  }                                                                                                    \
+  size_t zfp_decode_partial_block_strided_##N##_4(zfp_stream *z, T *b, size_t nx, size_t ny, size_t nz, \
+                                                  size_t nw, ptrdiff_t sx, ptrdiff_t sy, ptrdiff_t sz, \
+                                                  ptrdiff_t sw)                                        \
+  {                                                                                                    \
+    return block_partial(false, z, b, ZT, 4, nx, ny, nz, nw, sx, sy, sz, sw);                          \
+  }
+
+ZX_BLOCK_API(int32, int32, zfp_type_int32)
+ZX_BLOCK_API(int64, int64, zfp_type_int64)
+ZX_BLOCK_API(float, float, zfp_type_float)
+ZX_BLOCK_API(double, double, zfp_type_double)
+#undef ZX_BLOCK_API
 
 void zfp_stream_rewind(zfp_stream *z)
 {
