@@ -2476,6 +2476,18 @@ zfp_hip_batch *zfp_hip_batch_plan(int encode, zfp_stream *const *streams, zfp_fi
       }
       D.slice = spick;
     }
+    // the tuning runs wrote each variable-rate stream's words and offset
+    // index as a run does: keep the index's description, as a run would, so
+    // that a decompress planned before the first run reads through it
+    if (S.encode) {
+      for (const BatchItem &it : S.items) {
+        bitstream *s = streams[it.input]->stream;
+        if (it.job.fixed || !it.job.index_out || s->index != it.job.index_out) continue;
+        s->index_n = it.plan.nblocks;
+        s->index_start = it.start;
+        s->index_key = cfg_key(it.job.cfg, it.plan);
+      }
+    }
   }
   P->tuned_ms = best;
   // capture the device work once, on a private stream
