@@ -74,9 +74,10 @@ size_t zfp_hip_decompress_batch(zfp_stream *const *streams, zfp_field *const *fi
  * zfp_hip_batch_plan: plans the arrays (encode != 0: compress, else
  * decompress) from the streams' current cursors, tunes the launches' tiling
  * (blocks per workgroup, and the LDS slice of large fixed-rate arrays) by
- * timing the whole batch a few times -- those runs write what a run writes --
- * unless ZFP_HIP_BATCH_NO_TUNE is set, and captures the device work into a
- * graph.  Returns NULL unless every array is one the batched launches take
+ * timing the whole batch a few times -- those runs write what a run writes,
+ * variable-rate offset indexes included, so a decompress planned next reads
+ * through them -- unless ZFP_HIP_BATCH_NO_TUNE is set, and captures the
+ * device work into a graph.  Returns NULL unless every array is one the batched launches take
  * (device data and stream words, one array per stream, at least one block).
  * zfp_hip_batch_run: rewinds each stream to its planned cursor, replays the
  * graph on the first stream's HIP stream, and leaves cursors and sizes as the
