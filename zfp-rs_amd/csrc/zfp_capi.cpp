@@ -1109,9 +1109,52 @@ struct BatchItem {
 };
 
 // Grow-only pinned and device buffers of the batch entry points.
+// Side streams that a batch's independent class launches are spread over
+// (forked from and joined back into the caller's stream by events), so that
+// the many small launches of a mixed batch run beside each other and beside
+// the large ones.  Default priority, non-blocking.
+constexpr int kLanes = 3;
+struct Lanes {
+  hipStream_t s[kLanes] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kLanes] = {};
+  bool ready = false;
+
+  bool init()
+  {
+    if (ready) return true;
+    for (int i = 0; i < kLanes; i++)
+      if (hipStreamCreateWithFlags(&s[i], hipStreamNonBlocking) != hipSuccess) return fail();
+    if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return fail();
+    for (int i = 0; i < kLanes; i++)
+      if (hipEventCreateWithFlags(&join[i], hipEventDisableTiming) != hipSuccess) return fail();
+    ready = true;
+    return true;
+  }
+  bool fail()
+  {
+    (void)hipGetLastError();
+    release();
+    return false;
+  }
+  void release()
+  {
+    for (int i = 0; i < kLanes; i++) {
+      if (s[i]) (void)hipStreamDestroy(s[i]);
+      if (join[i]) (void)hipEventDestroy(join[i]);
+      s[i] = nullptr;
+      join[i] = nullptr;
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    fork = nullptr;
+    ready = false;
+  }
+};
+
 struct BatchBuffers {
   void *hbuf = nullptr; size_t hcap = 0;  // pinned: job table + firsts, then results
   void *dbuf = nullptr; size_t dcap = 0;  // device copy of the same
+  Lanes lanes;                             // for the synchronous batch calls
 };
 
 BatchBuffers &batch_buffers()
@@ -1165,6 +1208,7 @@ struct BatchDev {
   void *lens = nullptr, *offs = nullptr, *tiles = nullptr;
   uint32_t threads = 256;  // blocks per workgroup of the class launches
   uint32_t slice = 0;      // blocks per workgroup of arrays launched alone (0: default)
+  const Lanes *lanes = nullptr;  // side streams for the class launches, or none
 };
 
 // Plans the arrays; sizes[] is zeroed.  Arrays the batch does not take
@@ -1318,7 +1362,8 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
   uint32_t *lens = (uint32_t *)D.lens;
   uint64_t *offs = (uint64_t *)D.offs;
   const auto &items = S.items;
-  auto run = [&](bool var_only, bool count_pass) -> bool {
+  const Lanes *L = D.lanes && D.lanes->ready ? D.lanes : nullptr;
+  auto launches = [&](bool var_only, bool count_pass, auto &lane) -> bool {
     for (size_t k = 0; k < n;) {
       size_t e = k;
       uint64_t nb = 0;
@@ -1344,7 +1389,7 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
               ea.fixed = J.fixed;
               ea.vec = J.vec;
               ea.slice = D.slice;
-              r = launch_encode(t, dims, rev, ea, st);
+              r = launch_encode(t, dims, rev, ea, lane());
             } else {
               zx::DecodeArgs da{};
               da.field = J.field;
@@ -1356,7 +1401,7 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
               da.maxword = dres + 2 * q + 1;
               da.vec = J.vec;
               da.slice = D.slice;
-              r = launch_decode(t, dims, rev, da, nullptr, st);
+              r = launch_decode(t, dims, rev, da, nullptr, lane());
             }
             if (r != hipSuccess) {
               (void)hipGetLastError();
@@ -1374,7 +1419,8 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
         a.offsets = offs;
         a.threads = D.threads;
         const int v = cls >> 1;
-        const hipError_t r = encode ? kBEnc[v / 6][v % 6](a, count_pass, st) : kBDec[v / 6][v % 6](a, count_pass, st);
+        const hipStream_t ls = lane();
+        const hipError_t r = encode ? kBEnc[v / 6][v % 6](a, count_pass, ls) : kBDec[v / 6][v % 6](a, count_pass, ls);
         if (r != hipSuccess) {
           (void)hipGetLastError();
           return false;
@@ -1383,6 +1429,26 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
       k = e;
     }
     return true;
+  };
+  // launch i of a phase goes to stream i mod (kLanes + 1): the caller's, then
+  // the side streams, which start after the work before the phase and are
+  // joined back before the work after it (also when a launch fails)
+  auto run = [&](bool var_only, bool count_pass) -> bool {
+    const bool forked = L && hipEventRecord(L->fork, st) == hipSuccess;
+    uint32_t ready = 0;
+    while (forked && ready < (uint32_t)kLanes && hipStreamWaitEvent(L->s[ready], L->fork, 0) == hipSuccess)
+      ready++;
+    uint32_t launch = 0;
+    auto lane = [&]() -> hipStream_t {
+      const uint32_t i = launch++ % (kLanes + 1);
+      return i && i <= ready ? L->s[i - 1] : st;
+    };
+    const bool good = launches(var_only, count_pass, lane);
+    bool joined = true;
+    for (uint32_t i = 0; i < ready; i++)
+      joined = hipEventRecord(L->join[i], L->s[i]) == hipSuccess &&
+               hipStreamWaitEvent(st, L->join[i], 0) == hipSuccess && joined;
+    return good && joined && (!L || forked);
   };
   bool good = true;
   if (upload) good = hipMemcpyAsync(D.dbuf, D.hbuf, S.jbytes + S.fbytes, hipMemcpyHostToDevice, st) == hipSuccess;
@@ -1471,6 +1537,7 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
     D.lens = W.lens;
     D.offs = W.offs;
     D.tiles = W.tiles;
+    if (B.lanes.init()) D.lanes = &B.lanes;
     if (good) {
       batch_tables(S, D);
       good = batch_enqueue(S, D, true, st) && hipStreamSynchronize(st) == hipSuccess;
@@ -1556,11 +1623,13 @@ struct zfp_hip_batch {
   double tuned_ms = 0;                 // the chosen tiling's time at plan time
   SubmitBuffers *pooled = nullptr;     // a submission's buffers (returned to the pool)
   hipEvent_t done = nullptr;           // a submission's completion
+  Lanes lanes;                         // a plan's side streams
 
   ~zfp_hip_batch()
   {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    lanes.release();
     if (capture) (void)hipStreamDestroy(capture);
     if (done) (void)hipEventDestroy(done);
     if (pooled) {
@@ -2437,6 +2506,7 @@ zfp_hip_batch *zfp_hip_batch_plan(int encode, zfp_stream *const *streams, zfp_fi
   if (!P) return nullptr;
   BatchSet &S = P->S;
   BatchDev &D = P->D;
+  if (P->lanes.init()) D.lanes = &P->lanes;
   const hipStream_t st = hip_stream_of(streams[S.items[0].input]);
   if (hipMemcpyAsync(D.dbuf, D.hbuf, S.jbytes + S.fbytes, hipMemcpyHostToDevice, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess) {
