@@ -399,16 +399,13 @@ ZX_HD void chunk_to_coeffs(const typename PWord<SIZE>::T (&pl)[CH], int c,
         const int j = 32 * g + CH * f + i;
         if (j >= SIZE) continue;
         U &d = u[j < SIZE ? j : 0];
-        if constexpr (CH == 8) {
+        if constexpr (CH == 8 && sizeof(U) == 4) {
           // byte f of t[i] into byte c of the coefficient (empty until now)
-          const int h = (8 * c) / 32, k = c % 4;
-          const uint32_t w = (uint32_t)((uint64_t)d >> (32 * h));
-          const uint32_t r = perm_bytes(t[i], w, byte_insert_sel(k, 4 + f));
-          if constexpr (sizeof(U) == 8)
-            d = h ? (U)(((uint64_t)r << 32) | (uint32_t)(uint64_t)d) : (U)((((uint64_t)d >> 32) << 32) | r);
-          else
-            d = (U)r;
+          d = (U)perm_bytes(t[i], (uint32_t)d, byte_insert_sel(c, 4 + f));
         } else {
+          // (64-bit coefficients: an OR into the one dword the field lands
+          // in; rebuilding the coefficient from its halves kept two copies
+          // of every coefficient live across the chunks)
           d |= (U)((t[i] >> (CH * f)) & FM) << (CH * c);
         }
       }

@@ -27,6 +27,19 @@ namespace zx {
 
 constexpr int kThreads = 256;
 
+// Waves per SIMD the LDS kernels are held to: four for lossy 32-bit values
+// (128 VGPRs); none for 64-bit values, whose 64 coefficients alone take 128,
+// nor for reversible blocks, which would spill.
+template <class S, bool REV> constexpr int kMinWaves = ST<S>::P == 32 && !REV ? 4 : 1;
+
+// Waves per SIMD the decoders of 64-bit lossy blocks are held to: two (256
+// registers; the 64 coefficients take 128 and a few values spill), where
+// unconstrained they take about 270 and run one wave per SIMD, with nothing
+// to hide the latency of their loads.
+template <class S, bool REV> constexpr int kWideDecodeWaves = ST<S>::P == 64 && !REV ? 2 : 1;
+template <class S, bool REV>
+constexpr int kDecodeWaves = ST<S>::P == 32 ? kMinWaves<S, REV> : kWideDecodeWaves<S, REV>;
+
 // Fills the workgroup's expand8 table (zfpx_block.h); every kernel that
 // encodes calls it first, with all of its threads.
 __device__ __forceinline__ void expand8_fill()
@@ -128,7 +141,7 @@ __global__ __launch_bounds__(kThreads) void k_encode(EncodeArgs a)
 }
 
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) void k_decode(DecodeArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_decode(DecodeArgs a)
 {
   const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (b >= a.field.nblocks) return;
@@ -164,11 +177,6 @@ __global__ __launch_bounds__(64) void k_walk(DecodeArgs a, uint32_t *lengths)
 
 // ------------------------------------------------------------------ LDS path
 
-// Waves per SIMD the LDS kernels are held to: four for lossy 32-bit values
-// (128 VGPRs); none for 64-bit values, whose 64 coefficients alone take 128,
-// nor for reversible blocks, which would spill.
-template <class S, bool REV> constexpr int kMinWaves = ST<S>::P == 32 && !REV ? 4 : 1;
-
 // Fixed-rate encode through LDS (zfpx_fx.h): one workgroup per group of
 // blockDim.x blocks.  32-bit values fit four waves per SIMD without spilling
 // (the coder keeps packed chunk halves rather than the coefficients).
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // once per block): the group-test loop waits on LDS reads, and the extra wave
 // hides more of that than the spills cost.
 template <class S, int D, bool REV, bool FLAT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<S, REV>, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecodeWaves<S, REV>, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_encode(BatchArgs a)
 }
 
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) void k_batch_decode(BatchArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_batch_decode(BatchArgs a)
 {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nblocks) return;
