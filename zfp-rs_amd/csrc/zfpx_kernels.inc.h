@@ -40,6 +40,14 @@ template <class S, bool REV> constexpr int kWideDecodeWaves = ST<S>::P == 64 && 
 template <class S, bool REV>
 constexpr int kDecodeWaves = ST<S>::P == 32 ? kMinWaves<S, REV> : kWideDecodeWaves<S, REV>;
 
+// Waves per SIMD the encoders are held to: 32-bit as kMinWaves; 64-bit lossy
+// two (256 registers, a few values spilled), where unconstrained the
+// double-precision ones take a few more and run one wave per SIMD (reversible
+// double blocks would spill 60-150 bytes per lane under the cap).
+template <class S, bool REV> constexpr int kWideEncodeWaves = ST<S>::P == 64 && !REV ? 2 : 1;
+template <class S, bool REV>
+constexpr int kEncodeWaves = ST<S>::P == 32 ? kMinWaves<S, REV> : kWideEncodeWaves<S, REV>;
+
 // Fills the workgroup's expand8 table (zfpx_block.h); every kernel that
 // encodes calls it first, with all of its threads.
 __device__ __forceinline__ void expand8_fill()
@@ -120,7 +128,7 @@ __device__ __forceinline__ void store_block(const FieldDesc &f, uint64_t b, bool
 
 // Encode every block; COUNT writes per-block bit lengths instead of bits.
 template <class S, int D, bool REV, bool COUNT>
-__global__ __launch_bounds__(kThreads) void k_encode(EncodeArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideEncodeWaves<S, REV>, 8))) void k_encode(EncodeArgs a)
 {
   expand8_fill();
   const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -181,7 +189,7 @@ __global__ __launch_bounds__(64) void k_walk(DecodeArgs a, uint32_t *lengths)
 // blockDim.x blocks.  32-bit values fit four waves per SIMD without spilling
 // (the coder keeps packed chunk halves rather than the coefficients).
 template <class S, int D, bool REV, bool OWN>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<S, REV>, 8))) void k_encode_fx(EncodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kEncodeWaves<S, REV>, 8))) void k_encode_fx(EncodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   expand8_fill();
@@ -229,7 +237,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecod
 // its neighbours).  Denser workgroups are left to k_encode_dense.  The choice
 // is uniform over the workgroup, so the barriers are too.
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWaves<S, REV>, 8))) void k_encode_vx(EncodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kEncodeWaves<S, REV>, 8))) void k_encode_vx(EncodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
@@ -254,7 +262,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kMinWa
 // workgroup first tests blockDim.x groups at once (one per thread) and then
 // codes the dense ones among them.
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) void k_encode_dense(EncodeArgs a, FxPlan p, uint64_t ngroups)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideEncodeWaves<S, REV>, 8))) void k_encode_dense(EncodeArgs a, FxPlan p, uint64_t ngroups)
 {
   __shared__ uint32_t dense[kThreads];
   __shared__ uint32_t ndense;
@@ -302,7 +310,7 @@ __device__ __forceinline__ uint64_t batch_pos(const BatchArgs &a, const BatchJob
 
 // Every block of every array of one class; COUNT writes bit lengths.
 template <class S, int D, bool REV, bool COUNT>
-__global__ __launch_bounds__(kThreads) void k_batch_encode(BatchArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideEncodeWaves<S, REV>, 8))) void k_batch_encode(BatchArgs a)
 {
   expand8_fill();
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
