@@ -231,19 +231,35 @@ def test_variable_rate_sparse_and_dense_workgroups(ztype):
             check(arr, oc, 0, device=device, ctx=f"{name} device={device}")
 
 
-def test_buffer_too_small_matches_reference_words():
+@pytest.mark.parametrize("mode", ["rate", "precision", "accuracy", "reversible"])
+@pytest.mark.parametrize("device", [False, True])
+def test_buffer_too_small_matches_reference_words(mode, device):
     """An undersized buffer: compress reports failure, and the words that fit are the
-    reference's (writes past the end are dropped, core.rs:159-170)."""
-    arr = grf((16, 16, 16), seed=4)
-    oc = O.fixed_rate(8, O.FLOAT, 3)
+    reference's (writes past the end are dropped, core.rs:159-170), for fixed-rate and
+    variable-rate streams (the latter through the write pass's staged workgroups, whose
+    stores stop at the capacity)."""
+    arr = grf((16, 16, 24), seed=4)
+    if mode == "accuracy":
+        arr = arr.astype(np.float64) * 1e3
+    elif mode == "reversible":
+        arr = np.cumsum(np.round(arr * 8).astype(np.int32), axis=2, dtype=np.int32)
+    oc = {"rate": O.fixed_rate(8, O.FLOAT, 3), "precision": O.fixed_precision(20),
+          "accuracy": O.fixed_accuracy(1e-4), "reversible": O.reversible()}[mode]
     on, oerr, ow = O.compress(arr, oc)
-    short = on // 8 - 3
-    buf = np.zeros(short, dtype=np.uint64)
-    s = Z.Stream(buf, zcfg(oc))
-    with pytest.raises(Z.BufferTooSmall):
-        s.compress(arr)
-    assert np.array_equal(buf, ow[:short])
-    s.close()
+    assert oerr == 0
+    for short in (on // 8 - 3, on // 16):
+        if device:
+            buf = torch.zeros(short, dtype=torch.int64, device="cuda")
+            src = torch.from_numpy(arr).cuda()
+        else:
+            buf = np.zeros(short, dtype=np.uint64)
+            src = arr
+        s = Z.Stream(buf, zcfg(oc))
+        with pytest.raises(Z.BufferTooSmall):
+            s.compress(src)
+        got = buf.cpu().numpy().view(np.uint64) if device else buf
+        assert np.array_equal(got, ow[:short]), (mode, short)
+        s.close()
 
 
 def test_truncated_stream_is_reported():
