@@ -294,8 +294,15 @@ ZX_HD void tr_stage(uint32_t (&a)[N])
     ZX_UNROLL
     for (int i = k; i < k + J; i++) {
       uint32_t lo = a[i], hi = a[i + J];
-      a[i] = (lo & M) | ((hi << J) & ~M);
-      a[i + J] = ((lo >> J) & M) | (hi & ~M);
+      if constexpr (J == 8) {
+        // byte interleaves, one v_perm each: {lo.b0, hi.b0, lo.b2, hi.b2}
+        // and {lo.b1, hi.b1, lo.b3, hi.b3}
+        a[i] = perm_bytes(hi, lo, 0x06020400u);
+        a[i + J] = perm_bytes(hi, lo, 0x07030501u);
+      } else {
+        a[i] = (lo & M) | ((hi << J) & ~M);
+        a[i + J] = ((lo >> J) & M) | (hi & ~M);
+      }
     }
   }
 }
@@ -710,14 +717,15 @@ ZX_HD constexpr Expand8Table make_expand8_table()
 static constexpr Expand8Table kExpand8Host = make_expand8_table();
 
 #if ZX_DEVICE_BUILD
-// The table in device memory, and the workgroup's copy in LDS (32-bit
-// entries: a plain dword load needs no masking), which kernels that encode
-// fill (expand8_fill) before any thread codes a block.
+// The table in device memory, and the workgroup's copy in LDS (64-bit
+// entries: a load gives the entry already widened for the 64-bit shift that
+// places it, with no register to zero), which kernels that encode fill
+// (expand8_fill) before any thread codes a block.
 static __constant__ Expand8Table kExpand8Dev = make_expand8_table();
-static __shared__ uint32_t zx_expand8_lds[256];
+static __shared__ uint64_t zx_expand8_lds[256];
 #endif
 
-ZX_HD uint32_t expand8_of(uint32_t b)
+ZX_HD uint64_t expand8_of(uint32_t b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
   return zx_expand8_lds[b];
