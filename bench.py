@@ -9,7 +9,9 @@ Also reported:
   roofline      the dominant kernel (the slower of encode and decode):
                 algorithmic bytes per launch (field bytes + stream bytes) /
                 its average duration from HIP events recorded on the stream
-                the kernel runs on, against MI355X's 8 TB/s HBM peak;
+                the kernel runs on, against MI355X's 8 TB/s HBM peak, and
+                against the measured ceiling of a device-to-device copy of
+                the field (copy_ceiling, frac_of_copy);
   cpu_baseline  the oracle's threaded CPU codec (C-OMP chunked compress,
                 per-chunk-seek fixed-rate decompress) on a 512x512x64 slab of
                 the same field, timed on this host, and its stream checked
@@ -529,6 +531,18 @@ def main():
         out.copy_(x)
         torch.cuda.synchronize()
 
+    # the achievable ceiling: a device-to-device copy of the field (the same
+    # bytes read and written as a step's worth of one direction), best of 5
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    copy_ms = None
+    for _ in range(5):
+        ev0.record()
+        out.copy_(x)
+        ev1.record()
+        ev1.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        copy_ms = ms if copy_ms is None else min(copy_ms, ms)
+
     field_bytes = x.numel() * 4
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = 2 * field_bytes / (ms_per_step * 1e-3) / 1e9
@@ -571,7 +585,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic[0] if traffic else None,
                      "traffic_source": traffic[1] if traffic else None,
-                     "algorithmic_bytes_per_launch": alg},
+                     "algorithmic_bytes_per_launch": alg,
+                     "copy_ceiling": round(2 * field_bytes / (copy_ms * 1e-3) / 1e9, 1),
+                     "frac_of_copy": round(achieved / (2 * field_bytes / (copy_ms * 1e-3) / 1e9), 4)},
         "cpu_baseline": None,
     }
 
