@@ -21,7 +21,8 @@ Also reported:
                 p50/p99 latency, and the same work as one call per array;
   serving       Poisson arrivals (1000 req/s, 16 KiB-256 MiB arrays) through
                 the continuous-batching loop and one call per request:
-                GB/s and latency p50/p99;
+                GB/s and latency p50/p99; the same trace at 200 and 2000
+                req/s, and with every request arriving at once;
   variable_rate the variable-rate configurations (384^3 f64 accuracy and
                 precision, 8192^2 f32 precision, 256^3 i32 reversible):
                 compress and decompress GB/s through the offset index.
@@ -356,6 +357,27 @@ def serving_leg(n_requests, rate, seed=2000):
     replay(lambda: serve.serve_one_at_a_time(trace, codec))
     single = serve.summary(trace)
     note(f"one call per request: {single}")
+    # the same trace at the other ends of the arrival-rate range (arrival
+    # times scaled), batched and one call per request
+    sweep = []
+    for r_rps in (200.0, 2000.0):
+        scaled = [a * rate / r_rps for a in arrivals]
+
+        def replay_at(fn):
+            for r, a in zip(trace, scaled):
+                r.stream.rewind()
+                r.done = None
+                r.arrival = a
+            torch.cuda.synchronize()
+            fn()
+            torch.cuda.synchronize()
+
+        replay_at(lambda: serve.serve_batched(trace, codec, budget))
+        sb = serve.summary(trace)
+        replay_at(lambda: serve.serve_one_at_a_time(trace, codec))
+        ss = serve.summary(trace)
+        sweep.append({"rate_rps": r_rps, "batched": sb, "per_request": ss})
+        note(f"{r_rps:g} req/s: batched {sb}, one call per request {ss}")
     replay(lambda: serve.serve_batched(trace, codec, budget), at_once=True)
     sat_b = serve.summary(trace)
     replay(lambda: serve.serve_batched(trace, codec, budget, inflight=2), at_once=True)
@@ -370,7 +392,7 @@ def serving_leg(n_requests, rate, seed=2000):
     torch.cuda.empty_cache()
     return {"rate_rps": rate, "requests": n_requests, "uncompressed_bytes": total,
             "hbm_budget_bytes": budget, "batched": batched, "overlapped": overlapped,
-            "per_request": single,
+            "per_request": single, "rate_sweep": sweep,
             "saturated": {"batched_GBps": sat_b["GBps"], "overlapped_GBps": sat_o["GBps"],
                           "per_request_GBps": sat_s["GBps"],
                           "speedup": round(max(sat_b["GBps"], sat_o["GBps"]) / sat_s["GBps"], 2),

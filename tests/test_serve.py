@@ -24,6 +24,7 @@ class FakeCodec:
     def __init__(self, clock):
         self.clock = clock
         self.batches = []
+        self.singles = 0
 
     def run(self, compress, decompress):
         reqs = compress + decompress
@@ -33,6 +34,8 @@ class FakeCodec:
             r.size = r.nbytes // 4
 
     def run_one(self, r):
+        self.batches.append([id(r)])
+        self.singles += 1
         self.clock.t += 2e-4 + r.nbytes / 1e9
         r.size = r.nbytes // 4
 
@@ -66,6 +69,17 @@ def test_requests_arriving_during_a_batch_join_the_next():
     assert all(a[1] < b[0] for a, b in zip(spans, spans[1:]))
     s = serve.summary(trace)
     assert s["requests"] == 40 and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_a_lone_request_takes_the_single_array_call():
+    """A batch of one request skips the batch call's planning: the loop codes
+    it with the single-array call."""
+    clock = FakeClock()
+    codec = FakeCodec(clock)
+    trace = make_trace(6, gap=0.1)  # far apart: each arrives to an idle loop
+    stats = serve.serve_batched(trace, codec, budget=10 ** 12, clock=clock, sleep=clock.sleep)
+    assert stats["batches"] == 6 and codec.singles == 6
+    assert all(r.size == r.nbytes // 4 and r.done >= r.arrival for r in trace)
 
 
 def test_budget_bounds_the_admitted_bytes():

@@ -7,7 +7,9 @@ requests stay within a budget (the HBM the server may use), runs them as one
 batched compress and one batched decompress, and stamps each request's
 completion.  Requests that arrive while a batch is on the GPU join the next
 batch as soon as it returns, so the device always works on whatever has
-arrived; a request too large to share the budget runs alone.
+arrived; a request too large to share the budget runs alone.  A batch of one
+request (the common case at low load) goes through the single-array call,
+which skips the batch call's planning and table upload.
 
 With `inflight` > 1 and a codec that can submit without waiting, the loop
 keeps up to that many batches on the device at once (each on its own HIP
@@ -128,8 +130,13 @@ def serve_batched(trace, codec, budget, clock=time.perf_counter, sleep=time.slee
             sleep(max(0.0, todo[0].arrival - now))
             continue
         batch, used = admit(pending, budget)
-        codec.run([r for r in batch if r.kind == "compress"],
-                  [r for r in batch if r.kind == "decompress"])
+        if len(batch) == 1 and hasattr(codec, "run_one"):
+            # a lone request: the single-array call, without the batch
+            # call's planning and table upload
+            codec.run_one(batch[0])
+        else:
+            codec.run([r for r in batch if r.kind == "compress"],
+                      [r for r in batch if r.kind == "decompress"])
         end = clock() - t0
         for r in batch:
             r.done = end
@@ -157,6 +164,14 @@ def _serve_overlapped(trace, codec, budget, clock, sleep, inflight, poll=20e-6):
             pending.append(todo.popleft())
         if pending and len(flying) < inflight and (not flying or pending[0].footprint <= budget - used):
             batch, nbytes = admit(pending, budget - used if flying else budget)
+            if len(batch) == 1 and not flying and hasattr(codec, "run_one"):
+                # a lone request on an idle device: the single-array call
+                codec.run_one(batch[0])
+                batch[0].done = clock() - t0
+                stats["batches"] += 1
+                stats["max_batch"] = max(stats["max_batch"], 1)
+                stats["peak_bytes"] = max(stats["peak_bytes"], nbytes)
+                continue
             ticket = codec.submit([r for r in batch if r.kind == "compress"],
                                   [r for r in batch if r.kind == "decompress"], slot)
             slot = (slot + 1) % inflight
