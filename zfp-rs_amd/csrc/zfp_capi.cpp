@@ -19,7 +19,7 @@
 #include <algorithm>
 #include <memory>
 #include <mutex>
-#include <unordered_set>
+#include <atomic>
 #include <vector>
 
 #include "../../include/zfp.h"
@@ -107,6 +107,9 @@ struct bitstream {
   size_t index_cap;
   uint64_t index_start;
   uint64_t index_key;
+  // the batch call that last took this stream (duplicates within one call
+  // go to the single-array calls)
+  uint64_t batch_epoch;
 
   uint64_t get(size_t i) const
   {
@@ -1219,8 +1222,10 @@ void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, siz
                    size_t *sizes, uint flags, BatchSet &S)
 {
   S.encode = encode;
-  std::unordered_set<const bitstream *> seen;
-  S.items.reserve(count);
+  static std::atomic<uint64_t> epochs{0};
+  const uint64_t epoch = ++epochs;
+  std::vector<BatchItem> items;
+  items.reserve(count);
   for (size_t i = 0; i < count; i++) {
     sizes[i] = 0;
     zfp_stream *z = zs[i];
@@ -1228,10 +1233,11 @@ void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, siz
     Plan p;
     if (!batch_plan(z, f, !encode, p)) continue;
     bitstream *s = z->stream;
-    if (!seen.insert(s).second) {
+    if (s->batch_epoch == epoch) {
       S.later.push_back(i);
       continue;
     }
+    s->batch_epoch = epoch;
     if (p.nblocks == 0) {  // nothing to code: the calls' own handling
       S.later.push_back(i);
       continue;
@@ -1286,11 +1292,19 @@ void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, siz
     }
     const int variant = ((int)p.type - (int)zfp_type_int32) * 6 + (p.dims - 1) * 2 + (rev ? 1 : 0);
     it.cls = fixed && p.nblocks >= kSoloBlocks ? kSoloClass + variant : variant * 2 + (fixed ? 0 : 1);
-    S.items.push_back(it);
+    items.push_back(it);
   }
-  if (S.items.empty()) return;
-  std::stable_sort(S.items.begin(), S.items.end(),
-                   [](const BatchItem &a, const BatchItem &b) { return a.cls < b.cls; });
+  if (items.empty()) return;
+  // items grouped by class, in input order within a class: a counting sort
+  // over the class ranks (48 shared classes, then 24 of arrays launched
+  // alone), each item moved once
+  constexpr int kRanks = 48 + 24;
+  auto rank = [](int cls) { return cls >= kSoloClass ? 48 + (cls - kSoloClass) : cls; };
+  size_t at[kRanks + 1] = {};
+  for (const BatchItem &it : items) at[rank(it.cls) + 1]++;
+  for (int r = 0; r < kRanks; r++) at[r + 1] += at[r];
+  S.items.resize(items.size());
+  for (BatchItem &it : items) S.items[at[rank(it.cls)]++] = it;
   const size_t n = S.items.size();
   S.firsts.assign(n, 0);
   S.fill_first.assign(n, 0);
