@@ -1379,6 +1379,12 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
   uint64_t *offs = (uint64_t *)D.offs;
   const auto &items = S.items;
   const Lanes *L = D.lanes && D.lanes->ready ? D.lanes : nullptr;
+  // every variable-rate array of items [k, e) has its offset index
+  auto indexed = [&](size_t k, size_t e) {
+    for (size_t q = k; q < e; q++)
+      if (!items[q].job.fixed && !items[q].job.index) return false;
+    return true;
+  };
   auto launches = [&](bool var_only, bool count_pass, auto &lane) -> bool {
     for (size_t k = 0; k < n;) {
       size_t e = k;
@@ -1425,7 +1431,9 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
             }
           }
         }
-      } else if (!var_only || (cls & 1)) {
+      } else if ((!var_only || (cls & 1)) && !(var_only && !encode && indexed(k, e))) {
+        // (a decode's walk is launched only for a class with an array that
+        // has no offset index)
         zx::BatchArgs a{};
         a.jobs = djobs + k;
         a.firsts = dfirsts + k;
