@@ -17,7 +17,7 @@
  *   config helpers        src/config.rs:85-102, 642-686
  *
  * Scope: 1-D to 3-D fields of int32/int64/float/double, every mode and
- * rounding.  4-D blocks are not restated (out of scope for this build).
+ * rounding.  4-D blocks are not restated (the GPU codec does not code them).
  */
 #ifndef ZFP_ORACLE_H
 #define ZFP_ORACLE_H
