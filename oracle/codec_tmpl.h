@@ -86,17 +86,19 @@ static void FN(lift_axis)(TINT *p, int size, int s, void (*lift)(TINT *, int))
       lift(p + i, s);
 }
 
-/* forward: x, then y, then z (transform.rs:173-190, 212-216) */
+/* forward: x, then y, then z, then w (transform.rs:173-216) */
 static void FN(fwd_xform)(TINT *p, int size, void (*lift)(TINT *, int))
 {
   FN(lift_axis)(p, size, 1, lift);
   if (size >= 16) FN(lift_axis)(p, size, 4, lift);
   if (size >= 64) FN(lift_axis)(p, size, 16, lift);
+  if (size >= 256) FN(lift_axis)(p, size, 64, lift);
 }
 
-/* inverse: z, then y, then x (transform.rs:193-210) */
+/* inverse: w, then z, then y, then x (transform.rs:193-216) */
 static void FN(inv_xform)(TINT *p, int size, void (*lift)(TINT *, int))
 {
+  if (size >= 256) FN(lift_axis)(p, size, 64, lift);
   if (size >= 64) FN(lift_axis)(p, size, 16, lift);
   if (size >= 16) FN(lift_axis)(p, size, 4, lift);
   FN(lift_axis)(p, size, 1, lift);
@@ -156,19 +158,26 @@ static uint32_t FN(top)(const TUINT *u, int size)
   return acc ? (uint32_t)W - (uint32_t)orc_clz64((uint64_t)acc) + (64u - W) : 0u;
 }
 
-/* planes[k] bit i = bit k of coefficient i */
-static void FN(to_planes)(const TUINT *u, int size, uint64_t planes[64])
+/* planes[k][c] bit i = bit k of coefficient 64 c + i (a 4-D block's 256
+ * coefficients span four words, bitplane.rs:161-218) */
+static void FN(to_planes)(const TUINT *u, int size, uint64_t planes[64][4])
 {
-  for (int i = 0; i < 64; i++) planes[i] = i < size ? (uint64_t)u[i] : 0;
-  orc_transpose64(planes);
+  for (int c = 0; c < (size + 63) / 64; c++) {
+    uint64_t a[64];
+    for (int i = 0; i < 64; i++) a[i] = 64 * c + i < size ? (uint64_t)u[64 * c + i] : 0;
+    orc_transpose64(a);
+    for (int k = 0; k < 64; k++) planes[k][c] = a[k];
+  }
 }
 
-static void FN(from_planes)(const uint64_t planes_in[64], int size, TUINT *u)
+static void FN(from_planes)(const uint64_t planes[64][4], int size, TUINT *u)
 {
-  uint64_t a[64];
-  memcpy(a, planes_in, sizeof(a));
-  orc_transpose64(a);
-  for (int i = 0; i < size; i++) u[i] = (TUINT)a[i];
+  for (int c = 0; c < (size + 63) / 64; c++) {
+    uint64_t a[64];
+    for (int k = 0; k < 64; k++) a[k] = planes[k][c];
+    orc_transpose64(a);
+    for (int i = 0; i < 64 && 64 * c + i < size; i++) u[64 * c + i] = (TUINT)a[i];
+  }
 }
 
 /* encode_ints + encode_planes (bitplane.rs:509-610) */
@@ -178,7 +187,7 @@ static uint32_t FN(encode_ints)(orc_stream *s, uint32_t maxbits, uint32_t maxpre
   const uint32_t budget = orc_with_maxbits(maxbits, maxprec, (uint32_t)size) ? maxbits : UINT32_MAX;
   const uint32_t kmin = W > maxprec ? W - maxprec : 0;
   const uint32_t usize_ = (uint32_t)size;
-  uint64_t planes[64];
+  uint64_t planes[64][4];
   FN(to_planes)(u, size, planes);
   uint32_t top = FN(top)(u, size);
   uint32_t bits = budget;
@@ -192,24 +201,23 @@ static uint32_t FN(encode_ints)(orc_stream *s, uint32_t maxbits, uint32_t maxpre
     if (n == usize_) {
       while (bits > usize_ && k > kmin) {
         k--;
-        orc_put(s, planes[k] & orc_low_mask(usize_), usize_);
+        orc_put_plane(s, planes[k], usize_);
         bits -= usize_;
       }
       if (k == kmin) break;
     }
     k--;
-    uint64_t x = planes[k];
+    const uint64_t *x = planes[k];
     uint32_t m = n < bits ? n : bits;
     bits -= m;
-    orc_put(s, x & orc_low_mask(m), m);
+    orc_put_plane(s, x, m);
     while (bits != 0 && n < usize_) {
-      uint64_t rest = x >> n;
-      if (rest == 0) {
+      const uint32_t p = orc_next_one(x, n, usize_);
+      if (p == usize_) {
         orc_put(s, 0, 1);
         bits -= 1;
         break;
       }
-      uint32_t p = n + (uint32_t)orc_ctz64(rest);
       uint32_t zeros = p - n;
       int last = p == usize_ - 1;
       uint32_t len = zeros + 2 - (uint32_t)last;
@@ -238,7 +246,7 @@ static uint32_t FN(decode_ints)(orc_stream *s, uint32_t maxbits, uint32_t maxpre
   const uint32_t budget = constrained ? maxbits : UINT32_MAX;
   const uint32_t kmin = W > maxprec ? W - maxprec : 0;
   const uint32_t usize_ = (uint32_t)size;
-  uint64_t planes[64];
+  uint64_t planes[64][4];
   memset(planes, 0, sizeof(planes));
   uint32_t bits = budget, n = 0, m = 0, k = W, low = W;
   while (bits != 0) {
@@ -258,7 +266,7 @@ static uint32_t FN(decode_ints)(orc_stream *s, uint32_t maxbits, uint32_t maxpre
     } else if (n == usize_) {
       while (bits > usize_ && k > kmin) {
         k--;
-        planes[k] = orc_read_low(s, usize_);
+        orc_read_plane(s, planes[k], usize_);
         bits -= usize_;
         low = k;
       }
@@ -267,7 +275,8 @@ static uint32_t FN(decode_ints)(orc_stream *s, uint32_t maxbits, uint32_t maxpre
     k--;
     m = n < bits ? n : bits;
     bits -= m;
-    uint64_t x = orc_read_low(s, m);
+    uint64_t x[4];
+    orc_read_plane(s, x, m);
     while (bits != 0 && n < usize_) {
       uint64_t ahead = orc_peek(s);
       bits -= 1;
@@ -301,11 +310,11 @@ static uint32_t FN(decode_ints)(orc_stream *s, uint32_t maxbits, uint32_t maxpre
           orc_consume(s, span); bits -= span; n += span; lim -= span;
         }
       }
-      x |= 1ull << n;
+      x[n >> 6] |= 1ull << (n & 63);
       n += 1;
       m = n;
     }
-    planes[k] = x;
+    memcpy(planes[k], x, sizeof(x));
     low = k;
   }
   if (n == 0) low = W;
@@ -331,8 +340,8 @@ static uint32_t FN(encode_int_block)(orc_stream *s, const TINT *iblock_in, int s
                                      uint32_t minbits, uint32_t maxbits, uint32_t maxprec,
                                      int rounding)
 {
-  TINT block[64];
-  TUINT u[64];
+  TINT block[256];
+  TUINT u[256];
   memcpy(block, iblock_in, sizeof(TINT) * (size_t)size);
   FN(fwd_xform)(block, size, FN(fwd_lift));
   if (rounding == ORC_ROUND_FIRST) FN(fwd_round)(block, size, maxprec);
@@ -344,7 +353,7 @@ static uint32_t FN(encode_int_block)(orc_stream *s, const TINT *iblock_in, int s
 static uint32_t FN(decode_int_block)(orc_stream *s, TINT *iblock, int size, uint32_t minbits,
                                      uint32_t maxbits, uint32_t maxprec, int rounding)
 {
-  TUINT u[64];
+  TUINT u[256];
   int zero;
   uint32_t bits = FN(decode_ints)(s, maxbits, maxprec, rounding == ORC_ROUND_LAST, u, size, &zero);
   orc_skip_to(s, bits, minbits);
@@ -440,7 +449,7 @@ static uint32_t FN(encode_float_block)(orc_stream *s, const TFLOAT *f, int size,
     const uint32_t hb = EBITS + 1;
     uint32_t bmin = orc_sat_sub(c->minbits, hb), bmax = orc_sat_sub(c->maxbits, hb);
     orc_write_bits(s, 2 * (uint64_t)e + 1, hb);
-    TINT iblock[64];
+    TINT iblock[256];
     FN(fwd_cast)(iblock, f, size, emax);
     return hb + FN(encode_int_block)(s, iblock, size, bmin, bmax, prec, c->rounding);
   }
@@ -458,7 +467,7 @@ static uint32_t FN(decode_float_block)(orc_stream *s, TFLOAT *f, int size, int d
     int emax = (int)orc_read_bits(s, EBITS) - EBIAS;
     uint32_t prec = FN(precision)(emax, c->maxprec, c->minexp, dims, c->tight);
     uint32_t bmin = orc_sat_sub(c->minbits, hb), bmax = orc_sat_sub(c->maxbits, hb);
-    TINT iblock[64];
+    TINT iblock[256];
     uint32_t b = FN(decode_int_block)(s, iblock, size, bmin, bmax, prec, c->rounding);
     FN(inv_cast)(iblock, f, size, emax);
     return hb + b;
@@ -479,7 +488,7 @@ static uint32_t FN(rev_precision)(const TUINT *u, int size)
 static uint32_t FN(rev_encode_ints)(orc_stream *s, const TINT *iblock, int size,
                                     uint32_t maxbits, uint32_t maxprec)
 {
-  TUINT u[64];
+  TUINT u[256];
   FN(fwd_order)(u, iblock, orc_perm(size), size);
   uint32_t prec = FN(rev_precision)(u, size);
   if (prec > maxprec) prec = maxprec;
@@ -494,7 +503,7 @@ static uint32_t FN(rev_decode_ints)(orc_stream *s, TINT *iblock, int size, uint3
 {
   uint32_t prec = (uint32_t)orc_read_bits(s, PBITS) + 1;
   uint32_t mb = orc_sat_sub(maxbits, PBITS);
-  TUINT u[64];
+  TUINT u[256];
   int zero;
   uint32_t ub = FN(decode_ints)(s, mb, prec, 0, u, size, &zero);
   FN(inv_order)(u, iblock, orc_perm(size), size);
@@ -504,7 +513,7 @@ static uint32_t FN(rev_decode_ints)(orc_stream *s, TINT *iblock, int size, uint3
 /* integer reversible encode (encode/reversible.rs:286-300) */
 static uint32_t FN(rev_encode_int)(orc_stream *s, const TINT *in, int size, const orc_config *c)
 {
-  TINT block[64];
+  TINT block[256];
   memcpy(block, in, sizeof(TINT) * (size_t)size);
   FN(fwd_xform)(block, size, FN(rev_fwd_lift));
   uint32_t bits = FN(rev_encode_ints)(s, block, size, c->maxbits, c->maxprec);
@@ -525,7 +534,7 @@ static uint32_t FN(rev_encode_float)(orc_stream *s, const TFLOAT *f, int size, c
   int emax = FN(exponent_block)(f, size);
   uint32_t e = (uint32_t)(emax + EBIAS);
   int tiny = e != 0 && emax < MINCAST;
-  TINT iblock[64];
+  TINT iblock[256];
   for (int i = 0; i < size; i++) iblock[i] = 0;
   if (e != 0 && !tiny) FN(fwd_cast)(iblock, f, size, emax);
   int reversible = 0;
@@ -554,7 +563,7 @@ static uint32_t FN(rev_encode_float)(orc_stream *s, const TFLOAT *f, int size, c
       bits = 1;
     }
   } else {
-    TINT tc[64];
+    TINT tc[256];
     for (int i = 0; i < size; i++) {
       TUINT b;
       memcpy(&b, &f[i], sizeof(b));
@@ -578,7 +587,7 @@ static uint32_t FN(rev_decode_float)(orc_stream *s, TFLOAT *f, int size, const o
   uint32_t bits = 1;
   uint64_t reinterp = orc_read_bits(s, 1);
   bits += 1;
-  TINT iblock[64];
+  TINT iblock[256];
   if (reinterp) {
     bits += FN(rev_decode_ints)(s, iblock, size, orc_sat_sub(c->maxbits, bits));
     FN(inv_xform)(iblock, size, FN(rev_inv_lift));

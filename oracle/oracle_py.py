@@ -235,15 +235,15 @@ def block_offsets(words, shape, dtype, cfg, start_bits=0):
 def encode_block(st, cfg, ztype, dims, buf, origin, strides, lengths):
     """Encode one block from numpy buffer `buf` at element offset `origin`."""
     esz = buf.dtype.itemsize
-    s = (ctypes.c_ssize_t * 3)(*(list(strides) + [0] * (3 - len(strides))))
-    n = (ctypes.c_size_t * 3)(*(list(lengths) + [1] * (3 - len(lengths))))
+    s = (ctypes.c_ssize_t * 4)(*(list(strides) + [0] * (4 - len(strides))))
+    n = (ctypes.c_size_t * 4)(*(list(lengths) + [1] * (4 - len(lengths))))
     return lib().orc_encode_block(ctypes.byref(st.s), ctypes.byref(cfg), ztype, dims,
                                   buf.ctypes.data + origin * esz, s, n)
 
 
 def decode_block(st, cfg, ztype, dims, buf, origin, strides, lengths):
     esz = buf.dtype.itemsize
-    s = (ctypes.c_ssize_t * 3)(*(list(strides) + [0] * (3 - len(strides))))
-    n = (ctypes.c_size_t * 3)(*(list(lengths) + [1] * (3 - len(lengths))))
+    s = (ctypes.c_ssize_t * 4)(*(list(strides) + [0] * (4 - len(strides))))
+    n = (ctypes.c_size_t * 4)(*(list(lengths) + [1] * (4 - len(lengths))))
     return lib().orc_decode_block(ctypes.byref(st.s), ctypes.byref(cfg), ztype, dims,
                                   buf.ctypes.data + origin * esz, s, n)

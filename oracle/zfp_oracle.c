@@ -287,6 +287,38 @@ static inline uint64_t orc_read_low(orc_stream *s, uint32_t n)
   return n < 64 ? v & ((1ull << n) - 1) : v;
 }
 
+/* Bit planes of up to 256 coefficients as four words, coefficient i at bit
+ * i % 64 of word i / 64 (bitplane.rs:161-218). */
+/* the low m bits of plane x, word by word */
+static void orc_put_plane(orc_stream *s, const uint64_t x[4], uint32_t m)
+{
+  for (int c = 0; m != 0; c++) {
+    const uint32_t t = m < 64 ? m : 64;
+    orc_put(s, x[c] & orc_low_mask(t), t);
+    m -= t;
+  }
+}
+
+/* read m bits into the low bits of x; the rest of x is zero */
+static void orc_read_plane(orc_stream *s, uint64_t x[4], uint32_t m)
+{
+  for (int c = 0; c < 4; c++) {
+    const uint32_t t = m < 64 ? m : 64;
+    x[c] = t ? orc_read_low(s, t) : 0;
+    m -= t;
+  }
+}
+
+/* the lowest one-bit of x at or above bit n, or size when there is none */
+static uint32_t orc_next_one(const uint64_t x[4], uint32_t n, uint32_t size)
+{
+  for (uint32_t c = n / 64; c * 64 < size; c++) {
+    const uint64_t v = c == n / 64 ? x[c] >> (n % 64) << (n % 64) : x[c];
+    if (v) return c * 64 + (uint32_t)orc_ctz64(v);
+  }
+  return size;
+}
+
 /* pad_to / skip_to (encode/core.rs:147-172) */
 static uint32_t orc_pad_to(orc_stream *s, uint32_t bits, uint32_t minbits)
 {
@@ -310,9 +342,26 @@ static const uint8_t PERM3[64] = {
   57, 60, 51, 15, 43, 46, 58, 61, 55, 31, 62, 59, 47, 63,
 };
 
+/* index(i,j,k,l) = i + 4 j + 16 k + 64 l (encode/core.rs:26-39) */
+static const uint8_t PERM4[256] = {
+  0, 1, 4, 16, 64, 5, 80, 17, 68, 65, 20, 2, 8, 32, 128, 84, 81, 69, 21, 6, 18, 66, 24, 72, 9,
+  96, 33, 36, 129, 132, 144, 3, 12, 48, 192, 85, 82, 70, 22, 73, 25, 88, 37, 100, 97, 148, 145,
+  133, 10, 160, 34, 136, 130, 40, 7, 19, 67, 28, 76, 13, 112, 49, 52, 193, 196, 208, 86, 89, 101,
+  149, 161, 137, 41, 134, 38, 164, 26, 152, 146, 104, 98, 74, 83, 71, 23, 77, 29, 92, 53, 116,
+  113, 212, 209, 197, 11, 35, 131, 44, 140, 14, 176, 50, 56, 194, 200, 224, 90, 165, 102, 153,
+  150, 105, 168, 162, 138, 42, 87, 93, 117, 213, 27, 75, 99, 39, 135, 147, 108, 45, 141, 156, 30,
+  78, 177, 180, 54, 114, 120, 57, 198, 210, 216, 201, 225, 228, 15, 240, 51, 204, 195, 60, 169,
+  166, 154, 106, 91, 103, 151, 109, 157, 94, 181, 118, 121, 214, 217, 229, 163, 139, 43, 142, 46,
+  172, 58, 184, 178, 232, 226, 202, 241, 205, 61, 199, 55, 244, 31, 220, 211, 124, 115, 79, 170,
+  167, 155, 107, 158, 110, 173, 122, 185, 182, 233, 230, 218, 95, 245, 119, 221, 215, 125, 242,
+  206, 62, 203, 59, 248, 47, 236, 227, 188, 179, 143, 171, 174, 186, 234, 246, 222, 126, 219,
+  123, 249, 111, 237, 231, 189, 183, 159, 252, 243, 207, 63, 175, 250, 187, 238, 235, 190, 253,
+  247, 223, 127, 254, 251, 239, 191, 255,
+};
+
 static const uint8_t *orc_perm(int size)
 {
-  return size == 4 ? PERM1 : size == 16 ? PERM2 : PERM3;
+  return size == 4 ? PERM1 : size == 16 ? PERM2 : size == 64 ? PERM3 : PERM4;
 }
 
 /* ======================= templated codec ======================= */
@@ -434,35 +483,40 @@ size_t orc_maximum_size(const orc_config *c, int type, const size_t dims[3])
  * four with n real values reads from i, from 0 (i == 3), else from n - 1. */
 static inline size_t pad_src(size_t i, size_t n) { return i < n ? i : (i == 3 ? 0 : n - 1); }
 
-static void gather(void *out, const void *data, int type, int dims, const ptrdiff_t s[3],
-                   const size_t n[3])
+static void gather(void *out, const void *data, int type, int dims, const ptrdiff_t s[4],
+                   const size_t n[4])
 {
   const size_t esz = (type == ORC_INT32 || type == ORC_FLOAT) ? 4 : 8;
   const char *base = (const char *)data;
   int size = 1 << (2 * dims);
   for (int i = 0; i < size; i++) {
     size_t x = (size_t)i & 3, y = ((size_t)i >> 2) & 3, z = ((size_t)i >> 4) & 3;
+    size_t w = ((size_t)i >> 6) & 3;
     ptrdiff_t off = (ptrdiff_t)pad_src(x, n[0]) * s[0];
     if (dims >= 2) off += (ptrdiff_t)pad_src(y, n[1]) * s[1];
     if (dims >= 3) off += (ptrdiff_t)pad_src(z, n[2]) * s[2];
+    if (dims >= 4) off += (ptrdiff_t)pad_src(w, n[3]) * s[3];
     memcpy((char *)out + (size_t)i * esz, base + off * (ptrdiff_t)esz, esz);
   }
 }
 
-static void scatter(const void *in, void *data, int type, int dims, const ptrdiff_t s[3],
-                    const size_t n[3])
+static void scatter(const void *in, void *data, int type, int dims, const ptrdiff_t s[4],
+                    const size_t n[4])
 {
   const size_t esz = (type == ORC_INT32 || type == ORC_FLOAT) ? 4 : 8;
   char *base = (char *)data;
   int size = 1 << (2 * dims);
   for (int i = 0; i < size; i++) {
     size_t x = (size_t)i & 3, y = ((size_t)i >> 2) & 3, z = ((size_t)i >> 4) & 3;
+    size_t w = ((size_t)i >> 6) & 3;
     if (x >= n[0]) continue;
     if (dims >= 2 && y >= n[1]) continue;
     if (dims >= 3 && z >= n[2]) continue;
+    if (dims >= 4 && w >= n[3]) continue;
     ptrdiff_t off = (ptrdiff_t)x * s[0];
     if (dims >= 2) off += (ptrdiff_t)y * s[1];
     if (dims >= 3) off += (ptrdiff_t)z * s[2];
+    if (dims >= 4) off += (ptrdiff_t)w * s[3];
     memcpy(base + off * (ptrdiff_t)esz, (const char *)in + (size_t)i * esz, esz);
   }
 }
@@ -470,9 +524,9 @@ static void scatter(const void *in, void *data, int type, int dims, const ptrdif
 /* ======================= block entry points ======================= */
 
 uint32_t orc_encode_block(orc_stream *s, const orc_config *c, int type, int dims,
-                          const void *data, const ptrdiff_t strides[3], const size_t lengths[3])
+                          const void *data, const ptrdiff_t strides[4], const size_t lengths[4])
 {
-  uint64_t blk[64];
+  uint64_t blk[256];
   int size = 1 << (2 * dims);
   uint64_t before = orc_wtell(s);
   gather(blk, data, type, dims, strides, lengths);
@@ -501,9 +555,9 @@ uint32_t orc_encode_block(orc_stream *s, const orc_config *c, int type, int dims
 }
 
 uint32_t orc_decode_block(orc_stream *s, const orc_config *c, int type, int dims, void *data,
-                          const ptrdiff_t strides[3], const size_t lengths[3])
+                          const ptrdiff_t strides[4], const size_t lengths[4])
 {
-  uint64_t blk[64];
+  uint64_t blk[256];
   int size = 1 << (2 * dims);
   uint64_t before = orc_rtell(s);
   int rev = orc_is_reversible(c);
@@ -536,7 +590,7 @@ uint32_t orc_decode_block(orc_stream *s, const orc_config *c, int type, int dims
 typedef struct plan {
   int dims, type;
   size_t n[3], nb[3], nblocks;
-  ptrdiff_t s[3];
+  ptrdiff_t s[4]; /* s[3] unused: fields are 1-D to 3-D */
   size_t esz;
 } plan;
 
@@ -560,7 +614,7 @@ static int make_plan(const orc_field *f, plan *p)
 }
 
 /* block geometry (field_plan.rs:128-200), relative to data = element [0,0,0] */
-static const char *block_origin(const plan *p, const orc_field *f, size_t b, size_t len[3])
+static const char *block_origin(const plan *p, const orc_field *f, size_t b, size_t len[4])
 {
   size_t ix = b % p->nb[0], iy = (b / p->nb[0]) % p->nb[1], iz = b / (p->nb[0] * p->nb[1]);
   size_t c[3] = {ix, iy, iz};
@@ -579,7 +633,7 @@ static void compress_range(orc_stream *s, const orc_config *c, const orc_field *
 {
   orc_config cc = *c;
   for (size_t b = b0; b < b1; b++) {
-    size_t len[3];
+    size_t len[4] = {1, 1, 1, 1};
     const char *o = block_origin(p, f, b, len);
     orc_encode_block(s, &cc, p->type, p->dims, o, p->s, len);
   }
@@ -589,7 +643,7 @@ static void decompress_range(orc_stream *s, const orc_config *c, const orc_field
                              const plan *p, size_t b0, size_t b1)
 {
   for (size_t b = b0; b < b1; b++) {
-    size_t len[3];
+    size_t len[4] = {1, 1, 1, 1};
     char *o = (char *)block_origin(p, f, b, len);
     orc_decode_block(s, c, p->type, p->dims, o, p->s, len);
   }
@@ -628,8 +682,8 @@ int orc_block_offsets(orc_stream *s, const orc_config *c, const orc_field *f, ui
   /* decode into a scratch copy of one block only */
   for (size_t b = 0; b < p.nblocks; b++) {
     uint64_t blk[64];
-    ptrdiff_t unit[3] = {1, 4, 16};
-    size_t full[3] = {4, 4, 4};
+    ptrdiff_t unit[4] = {1, 4, 16, 64};
+    size_t full[4] = {4, 4, 4, 4};
     offsets[b] = orc_rtell(s);
     orc_decode_block(s, c, p.type, p.dims, blk, unit, full);
   }
@@ -799,4 +853,9 @@ void orc_inv_round_u32(uint32_t *d, int size, uint32_t m, uint32_t prec) { inv_r
 void orc_inv_round_u64(uint64_t *d, int size, uint32_t m, uint32_t prec) { inv_round_64(d, size, m, prec); }
 
 /* planes[k] bit i = bit k of u[i] (u given as 64-bit values) */
-void orc_to_planes(const uint64_t *u, int size, uint64_t planes[64]) { to_planes_64(u, size, planes); }
+void orc_to_planes(const uint64_t *u, int size, uint64_t planes[64])
+{
+  uint64_t p4[64][4];
+  to_planes_64(u, size, p4);
+  for (int k = 0; k < 64; k++) planes[k] = p4[k][0];
+}

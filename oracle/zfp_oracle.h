@@ -10,14 +10,14 @@
  *   lifting transforms    src/codec/transform.rs:74-142, axis order :151-234
  *   block encode          src/codec/encode/{core,float,integer,reversible}.rs
  *   block decode          src/codec/decode/{core,reversible}.rs
- *   gathers / scatters    src/codec/encode/dim{1,2,3}.rs, src/codec/decode/dim{1,2,3}.rs
+ *   gathers / scatters    src/codec/encode/dim{1,2,3,4}.rs, src/codec/decode/dim{1,2,3,4}.rs
  *   field drivers         src/compress.rs:22-115, src/decompress.rs:27-147
  *   parallel drivers      src/compress.rs:132-236, src/decompress.rs:205-330,
  *                         src/execution.rs:119-148
  *   config helpers        src/config.rs:85-102, 642-686
  *
- * Scope: 1-D to 3-D fields of int32/int64/float/double, every mode and
- * rounding.  4-D blocks are not restated (the GPU codec does not code them).
+ * Scope: 1-D to 3-D fields and 1-D to 4-D blocks of int32/int64/float/double,
+ * every mode and rounding.
  */
 #ifndef ZFP_ORACLE_H
 #define ZFP_ORACLE_H
@@ -86,15 +86,16 @@ uint32_t orc_block_bits(const orc_config *c, int type, int dims);
 size_t orc_maximum_size(const orc_config *c, int type, const size_t dims[3]);
 int orc_is_reversible(const orc_config *c);
 
-/* ---- block codec: one 4^d block through strides ----
- * `data` points at the block origin; lengths nx..nz (1..4) give partial
- * blocks, padded as C's pad_block does.  Returns bits written/read. */
+/* ---- block codec: one 4^d block (d = 1..4) through strides ----
+ * `data` points at the block origin; lengths nx..nw (1..4) give partial
+ * blocks, padded as C's pad_block does.  Entries past `dims` are not read.
+ * Returns bits written/read. */
 uint32_t orc_encode_block(orc_stream *s, const orc_config *c, int type, int dims,
-                          const void *data, const ptrdiff_t strides[3],
-                          const size_t lengths[3]);
+                          const void *data, const ptrdiff_t strides[4],
+                          const size_t lengths[4]);
 uint32_t orc_decode_block(orc_stream *s, const orc_config *c, int type, int dims,
-                          void *data, const ptrdiff_t strides[3],
-                          const size_t lengths[3]);
+                          void *data, const ptrdiff_t strides[4],
+                          const size_t lengths[4]);
 
 /* ---- field drivers ----
  * compress: returns stream byte length after flush, or 0 with *err set

@@ -135,32 +135,38 @@ def lcg_block(ztype, n):
 
 
 def strided_array(ztype, dims, dummy=99):
-    """make_strided_array of encode_block_strided.rs (SX=2, SY=24, SZ=192)."""
+    """make_strided_array of encode_block_strided.rs (SX=2, SY=24, SZ=192,
+    SW=2304; the 4-D layout at :685-712)."""
     dtype = {1: np.int32, 2: np.int64, 3: np.float32, 4: np.float64}[ztype]
-    SX, SY, SZ = 2, 24, 192
+    SX, SY, SZ, SW = 2, 24, 192, 2304
     cx = 4 * SX
     cy = SY // SX if dims >= 2 else 1
     cz = SZ // SY if dims >= 3 else 1
-    data = np.full(cx * cy * cz, dummy, dtype=dtype)
+    cw = SW // SZ if dims >= 4 else 1
+    data = np.full(cx * cy * cz * cw, dummy, dtype=dtype)
     vals = iter(lcg_block(ztype, 4 ** dims))
-    for k in range(cz):
-        for j in range(cy):
-            for i in range(cx):
-                if i % 2 == 0 and (dims < 2 or j % (cy // 4) == 0) and (dims < 3 or k % (cz // 4) == 0):
-                    data[cx * cy * k + cx * j + i] = next(vals)
-    strides = [SX, SY, SZ][:dims]
+    for l in range(cw):
+        for k in range(cz):
+            for j in range(cy):
+                for i in range(cx):
+                    if (i % 2 == 0 and (dims < 2 or j % (cy // 4) == 0) and (dims < 3 or k % (cz // 4) == 0)
+                            and (dims < 4 or l % (cw // 4) == 0)):
+                        data[cx * cy * cz * l + cx * cy * k + cx * j + i] = next(vals)
+    strides = [SX, SY, SZ, SW][:dims]
     return data, strides
 
 
 def hash_strided(arr, dims, strides):
     """hashStridedArray over the 4^d strided positions (checksums.rs:79-151)."""
     idx = []
-    for z in range(4 if dims >= 3 else 1):
-        for y in range(4 if dims >= 2 else 1):
-            for x in range(4):
-                off = x * strides[0]
-                if dims >= 2: off += y * strides[1]
-                if dims >= 3: off += z * strides[2]
-                idx.append(off)
+    for w in range(4 if dims >= 4 else 1):
+        for z in range(4 if dims >= 3 else 1):
+            for y in range(4 if dims >= 2 else 1):
+                for x in range(4):
+                    off = x * strides[0]
+                    if dims >= 2: off += y * strides[1]
+                    if dims >= 3: off += z * strides[2]
+                    if dims >= 4: off += w * strides[3]
+                    idx.append(off)
     vals = arr[np.array(idx)]
     return hash_array32(vals) if arr.dtype.itemsize == 4 else hash_array64(vals)

@@ -17,7 +17,7 @@ import oracle_py as O
 from kat_util import (SPECIAL_F32, SPECIAL_F64, TNAMES, checksum, compute_key, hash_array32,
                       hash_array64, hash_bitstream, hash_strided, lcg_block, strided_array)
 
-CASES = [(d, t) for d in (1, 2, 3) for t in (1, 2, 3, 4)]
+CASES = [(d, t) for d in (1, 2, 3, 4) for t in (1, 2, 3, 4)]
 MODE_FIXED_RATE, MODE_REVERSIBLE = 2, 5
 
 
@@ -44,7 +44,7 @@ def test_seeded_input_matches(dims, ztype):
 def encode_full(dims, ztype):
     blk = lcg_block(ztype, 4 ** dims)
     st = O.OracleStream(65536)
-    bits = O.encode_block(st, rate_cfg(dims), ztype, dims, blk, 0, [1, 4, 16][:dims], [4] * dims)
+    bits = O.encode_block(st, rate_cfg(dims), ztype, dims, blk, 0, [1, 4, 16, 64][:dims], [4] * dims)
     assert bits == st.wtell()
     st.flush()
     return blk, st
@@ -62,13 +62,13 @@ def test_full_block_decode_matches(dims, ztype):
     blk, st = encode_full(dims, ztype)
     st.rewind()
     out = np.zeros_like(blk)
-    bits = O.decode_block(st, rate_cfg(dims), ztype, dims, out, 0, [1, 4, 16][:dims], [4] * dims)
+    bits = O.decode_block(st, rate_cfg(dims), ztype, dims, out, 0, [1, 4, 16, 64][:dims], [4] * dims)
     assert bits == 4 ** dims * 19
     k1, k2 = compute_key(0, 2, dim_lens(dims), MODE_FIXED_RATE, 0)
     assert hash_vals(out) == checksum(dims, TNAMES[ztype], k1, k2)
 
 
-PARTIAL = [1, 2, 3]
+PARTIAL = [1, 2, 3, 4]
 
 
 @pytest.mark.parametrize("dims,ztype", CASES)
@@ -94,7 +94,7 @@ def test_partial_strided_decode_matches(dims, ztype):
     assert hash_strided(out, dims, strides) == checksum(dims, TNAMES[ztype], k1, k2)
 
 
-@pytest.mark.parametrize("dims,ztype", [(d, t) for d in (1, 2, 3) for t in (3, 4)])
+@pytest.mark.parametrize("dims,ztype", [(d, t) for d in (1, 2, 3, 4) for t in (3, 4)])
 def test_reversible_special_blocks_match(dims, ztype):
     specials = SPECIAL_F32 if ztype == 3 else SPECIAL_F64
     dtype = np.float32 if ztype == 3 else np.float64
@@ -109,13 +109,13 @@ def test_reversible_special_blocks_match(dims, ztype):
         raw[::4] = bits
         blk = raw.view(dtype)
         st = O.OracleStream(65536)
-        O.encode_block(st, O.reversible(), ztype, dims, blk, 0, [1, 4, 16][:dims], [4] * dims)
+        O.encode_block(st, O.reversible(), ztype, dims, blk, 0, [1, 4, 16, 64][:dims], [4] * dims)
         st.flush()
         assert hash_bitstream(st.committed()) == want, f"special {i}"
         # and it decodes bit for bit
         st.rewind()
         out = np.zeros_like(blk)
-        O.decode_block(st, O.reversible(), ztype, dims, out, 0, [1, 4, 16][:dims], [4] * dims)
+        O.decode_block(st, O.reversible(), ztype, dims, out, 0, [1, 4, 16, 64][:dims], [4] * dims)
         assert np.array_equal(out.view(utype), raw), f"special {i} round trip"
         checked += 1
     assert checked >= 6
