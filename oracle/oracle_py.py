@@ -247,3 +247,39 @@ def decode_block(st, cfg, ztype, dims, buf, origin, strides, lengths):
     n = (ctypes.c_size_t * 4)(*(list(lengths) + [1] * (4 - len(lengths))))
     return lib().orc_decode_block(ctypes.byref(st.s), ctypes.byref(cfg), ztype, dims,
                                   buf.ctypes.data + origin * esz, s, n)
+
+
+def _blocks(shape):
+    """(origin, strides, lengths) of each block of a C-contiguous array of numpy
+    shape `shape`, in the reference's order: x fastest (field_plan.rs:41-77),
+    partial blocks at the high ends."""
+    n = list(reversed(shape))
+    nb = [(v + 3) // 4 for v in n]
+    strides = [int(np.prod(n[:a], dtype=np.int64)) for a in range(len(n))]
+    for idx in np.ndindex(*reversed(nb)):
+        c = list(reversed(idx))
+        origin = sum(4 * c[a] * strides[a] for a in range(len(n)))
+        yield origin, strides, [min(4, n[a] - 4 * c[a]) for a in range(len(n))]
+
+
+def compress_blocks(arr, cfg, nwords):
+    """The reference's serial compress loop (src/compress.rs:22-115) over the
+    block coder, for 1-D to 4-D arrays: (words after the flush, per-block start
+    bits plus the end)."""
+    st = OracleStream(words=np.zeros(nwords, dtype=np.uint64))
+    flat = np.ascontiguousarray(arr).reshape(-1)
+    offs = [0]
+    for origin, strides, lens in _blocks(arr.shape):
+        encode_block(st, cfg, TYPE_OF[arr.dtype], arr.ndim, flat, origin, strides, lens)
+        offs.append(st.wtell())
+    st.flush()
+    return st.words.copy(), np.array(offs, dtype=np.uint64)
+
+
+def decompress_blocks(words, shape, dtype, cfg):
+    """The serial decompress loop (src/decompress.rs:27-147) over the block decoder."""
+    out = np.zeros(int(np.prod(shape)), dtype=dtype)
+    st = OracleStream(words=np.ascontiguousarray(words, dtype=np.uint64).copy())
+    for origin, strides, lens in _blocks(shape):
+        decode_block(st, cfg, TYPE_OF[np.dtype(dtype)], len(shape), out, origin, strides, lens)
+    return out.reshape(shape)

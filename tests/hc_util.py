@@ -38,9 +38,9 @@ def lib():
 
 
 def _dims(arr):
-    n = list(reversed(arr.shape)) + [1] * (3 - arr.ndim)
-    s = [1, n[0], n[0] * n[1]]
-    return (ctypes.c_int64 * 3)(*n), (ctypes.c_int64 * 3)(*s)
+    n = list(reversed(arr.shape)) + [1] * (4 - arr.ndim)
+    s = [1, n[0], n[0] * n[1], n[0] * n[1] * n[2]]
+    return (ctypes.c_int64 * 4)(*n), (ctypes.c_int64 * 4)(*s)
 
 
 def cfg_of(oc):

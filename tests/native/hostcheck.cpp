@@ -14,11 +14,11 @@ struct HcCfg {
   int32_t minexp, rounding, tight;
 };
 
-static FieldDesc make_field(int dims, void *data, const int64_t n[3], const int64_t s[3])
+static FieldDesc make_field(int dims, void *data, const int64_t n[4], const int64_t s[4])
 {
   FieldDesc f;
   f.data = data;
-  for (int a = 0; a < 3; a++) {
+  for (int a = 0; a < 4; a++) {
     f.n[a] = a < dims ? n[a] : 1;
     f.s[a] = a < dims ? s[a] : 0;
   }
@@ -173,6 +173,7 @@ static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, 
                        uint64_t st, uint32_t fx, uint32_t *l, uint32_t T, uint32_t cap)
 {
   FxPlan p;
+  if (dims == 4) return compress_t<S, 4, REV>(f, c, w, nw, st, fx, l);  // global path only
   if (T && !fx) {
     switch (dims) {
     case 1: return compress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);
@@ -199,6 +200,10 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
                          uint64_t st, uint32_t fx, const uint32_t *l, uint32_t T)
 {
   FxPlan p;
+  if (dims == 4) {  // global path only
+    decompress_t<S, 4, REV>(f, c, w, nw, st, fx, l);
+    return 0;
+  }
   if (T && fx_plan(fx, st, true, T, p)) {
     switch (dims) {
     case 1: return decompress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);

@@ -694,24 +694,24 @@ typedef hipError_t (*EncFn)(const zx::EncodeArgs &, hipStream_t);
 typedef hipError_t (*DecFn)(const zx::DecodeArgs &, uint32_t *, hipStream_t);
 
 #define ZX_ROW(T) {zx_enc_##T##_1_0, zx_enc_##T##_1_1, zx_enc_##T##_2_0, zx_enc_##T##_2_1, \
-                   zx_enc_##T##_3_0, zx_enc_##T##_3_1}
-const EncFn kEnc[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
+                   zx_enc_##T##_3_0, zx_enc_##T##_3_1, zx_enc_##T##_4_0, zx_enc_##T##_4_1}
+const EncFn kEnc[4][8] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
 #undef ZX_ROW
 #define ZX_ROW(T) {zx_dec_##T##_1_0, zx_dec_##T##_1_1, zx_dec_##T##_2_0, zx_dec_##T##_2_1, \
-                   zx_dec_##T##_3_0, zx_dec_##T##_3_1}
-const DecFn kDec[4][6] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
+                   zx_dec_##T##_3_0, zx_dec_##T##_3_1, zx_dec_##T##_4_0, zx_dec_##T##_4_1}
+const DecFn kDec[4][8] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
 #undef ZX_ROW
 
 hipError_t launch_encode(zfp_type t, int dims, bool rev, const zx::EncodeArgs &a, hipStream_t s)
 {
-  if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 3) return hipErrorInvalidValue;
+  if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 4) return hipErrorInvalidValue;
   return kEnc[t - zfp_type_int32][2 * (dims - 1) + (rev ? 1 : 0)](a, s);
 }
 
 hipError_t launch_decode(zfp_type t, int dims, bool rev, const zx::DecodeArgs &a, uint32_t *walk,
                          hipStream_t s)
 {
-  if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 3) return hipErrorInvalidValue;
+  if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 4) return hipErrorInvalidValue;
   return kDec[t - zfp_type_int32][2 * (dims - 1) + (rev ? 1 : 0)](a, walk, s);
 }
 
@@ -729,7 +729,7 @@ zx::FieldDesc field_desc(const Plan &p, void *origin)
 {
   zx::FieldDesc f;
   f.data = origin;
-  for (int a = 0; a < 3; a++) {
+  for (int a = 0; a < 4; a++) {
     f.n[a] = p.n[a];
     f.s[a] = p.s[a];
   }
@@ -742,6 +742,7 @@ bool vec_ok(const Plan &p, const void *origin)
   if (p.s[0] != 1) return false;
   if (p.dims >= 2 && (p.s[1] % 4) != 0) return false;
   if (p.dims >= 3 && (p.s[2] % 4) != 0) return false;
+  if (p.dims >= 4 && (p.s[3] % 4) != 0) return false;
   return ((uintptr_t)origin % 16) == 0;
 }
 
@@ -1018,14 +1019,13 @@ bool block_plan(zfp_type t, int dims, const size_t len[4], const ptrdiff_t str[4
 // src/codec/block/strided.rs:420-700): the same coder as a one-block field,
 // with the cursor left right after the block (the writer holding the block's
 // last partial word, unflushed; the reader unaligned).  Returns the block's
-// bits, or 0 for a null block, a length outside 1..4, a 4-D block (not built
-// for the GPU) or a device failure.
+// bits, or 0 for a null block, a length outside 1..4 or a device failure.
 size_t block_code(bool encode, zfp_stream *z, const void *block, zfp_type t, int dims,
                   const size_t len[4], const ptrdiff_t str[4])
 {
   if (!z || !z->stream || !block) return 0;
   Plan p;
-  if (!block_plan(t, dims, len, str, p) || dims > 3) return 0;
+  if (!block_plan(t, dims, len, str, p)) return 0;
   if (((uintptr_t)((const char *)block + p.imin * (int64_t)p.esz)) % p.esz) return 0;
   zfp_field f{};
   f.type = t;
@@ -2049,7 +2049,6 @@ size_t zfp_compress(zfp_stream *z, const zfp_field *f)
   if (!z || !f || !z->stream || !f->data) return 0;
   Plan p;
   if (!make_plan(f, p)) return 0;
-  if (p.dims > 3) return 0;  // 4-D blocks are not built for the GPU
   if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return 0;
   bitstream *s = z->stream;
   const zx::Cfg c = cfg_of(z);
@@ -2069,7 +2068,6 @@ size_t zfp_decompress(zfp_stream *z, zfp_field *f)
   if (!z || !f || !z->stream || !f->data || !valid_type(f->type)) return 0;
   Plan p;
   if (!make_plan(f, p)) return 0;
-  if (p.dims > 3) return 0;
   if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return 0;
   bitstream *s = z->stream;
   const zx::Cfg c = cfg_of(z);

@@ -72,7 +72,8 @@ struct BatchArgs {
 }  // namespace zx
 
 // kernel launchers, one pair per (type, dimensionality, reversible):
-// zx_{enc,dec}_<f32|f64|i32|i64>_<1|2|3>_<0|1> (zfpx_kernels.hip)
+// zx_{enc,dec}_<f32|f64|i32|i64>_<1|2|3|4>_<0|1> (zfpx_kernels.hip); the batch
+// launchers of 4-D return hipErrorNotSupported
 #define ZX_DECLARE1(T, D, R)                                                              \
   hipError_t zx_enc_##T##_##D##_##R(const zx::EncodeArgs &a, hipStream_t s);              \
   hipError_t zx_dec_##T##_##D##_##R(const zx::DecodeArgs &a, uint32_t *walk, hipStream_t s); \
@@ -80,7 +81,7 @@ struct BatchArgs {
   hipError_t zx_bdec_##T##_##D##_##R(const zx::BatchArgs &a, bool walk, hipStream_t s);
 #define ZX_DECLARE(T)                                                                     \
   ZX_DECLARE1(T, 1, 0) ZX_DECLARE1(T, 1, 1) ZX_DECLARE1(T, 2, 0) ZX_DECLARE1(T, 2, 1)     \
-  ZX_DECLARE1(T, 3, 0) ZX_DECLARE1(T, 3, 1)
+  ZX_DECLARE1(T, 3, 0) ZX_DECLARE1(T, 3, 1) ZX_DECLARE1(T, 4, 0) ZX_DECLARE1(T, 4, 1)
 ZX_DECLARE(f32)
 ZX_DECLARE(f64)
 ZX_DECLARE(i32)

@@ -140,15 +140,30 @@ template <int D> struct Blk { static constexpr int SIZE = 1 << (2 * D); };
 
 // ---------------------------------------------------------------- permutation
 
-// PERM_1..3 (encode/core.rs:12-24): coefficient order by sequency.
+// PERM_1..4 (encode/core.rs:12-39): coefficient order by sequency.
 ZX_HD constexpr int perm(int d, int j)
 {
+  constexpr uint8_t p4[256] = {
+      0,   1,   4,   16,  64,  5,   80,  17,  68,  65,  20,  2,   8,   32,  128, 84,  81,  69,  21,
+      6,   18,  66,  24,  72,  9,   96,  33,  36,  129, 132, 144, 3,   12,  48,  192, 85,  82,  70,
+      22,  73,  25,  88,  37,  100, 97,  148, 145, 133, 10,  160, 34,  136, 130, 40,  7,   19,  67,
+      28,  76,  13,  112, 49,  52,  193, 196, 208, 86,  89,  101, 149, 161, 137, 41,  134, 38,  164,
+      26,  152, 146, 104, 98,  74,  83,  71,  23,  77,  29,  92,  53,  116, 113, 212, 209, 197, 11,
+      35,  131, 44,  140, 14,  176, 50,  56,  194, 200, 224, 90,  165, 102, 153, 150, 105, 168, 162,
+      138, 42,  87,  93,  117, 213, 27,  75,  99,  39,  135, 147, 108, 45,  141, 156, 30,  78,  177,
+      180, 54,  114, 120, 57,  198, 210, 216, 201, 225, 228, 15,  240, 51,  204, 195, 60,  169, 166,
+      154, 106, 91,  103, 151, 109, 157, 94,  181, 118, 121, 214, 217, 229, 163, 139, 43,  142, 46,
+      172, 58,  184, 178, 232, 226, 202, 241, 205, 61,  199, 55,  244, 31,  220, 211, 124, 115, 79,
+      170, 167, 155, 107, 158, 110, 173, 122, 185, 182, 233, 230, 218, 95,  245, 119, 221, 215, 125,
+      242, 206, 62,  203, 59,  248, 47,  236, 227, 188, 179, 143, 171, 174, 186, 234, 246, 222, 126,
+      219, 123, 249, 111, 237, 231, 189, 183, 159, 252, 243, 207, 63,  175, 250, 187, 238, 235, 190,
+      253, 247, 223, 127, 254, 251, 239, 191, 255};
   constexpr uint8_t p2[16] = {0, 1, 4, 5, 2, 8, 6, 9, 3, 12, 10, 7, 13, 11, 14, 15};
   constexpr uint8_t p3[64] = {
       0,  1,  4,  16, 20, 17, 5,  2,  8,  32, 21, 6,  18, 24, 9,  33, 36, 3,  12, 48, 22, 25,
       37, 40, 34, 10, 7,  19, 28, 13, 49, 52, 41, 38, 26, 23, 29, 53, 11, 35, 44, 14, 50, 56,
       42, 27, 39, 45, 30, 54, 57, 60, 51, 15, 43, 46, 58, 61, 55, 31, 62, 59, 47, 63};
-  return d == 1 ? j : d == 2 ? p2[j] : p3[j];
+  return d == 1 ? j : d == 2 ? p2[j] : d == 3 ? p3[j] : p4[j];
 }
 
 // ---------------------------------------------------------------- lifting
@@ -229,15 +244,17 @@ struct InvL { template <class I> ZX_HD void operator()(I &x, I &y, I &z, I &w) c
 struct RFwdL { template <class I> ZX_HD void operator()(I &x, I &y, I &z, I &w) const { rev_fwd_lift(x, y, z, w); } };
 struct RInvL { template <class I> ZX_HD void operator()(I &x, I &y, I &z, I &w) const { rev_inv_lift(x, y, z, w); } };
 
-// x, then y, then z (forward); z, y, x (inverse)
+// x, then y, then z, then w (forward); w, z, y, x (inverse)
 template <int D, class I, class L> ZX_HD void fwd_xform(I (&p)[Blk<D>::SIZE], L lift)
 {
   lift_axis<Blk<D>::SIZE, 1>(p, lift);
   if constexpr (D >= 2) lift_axis<Blk<D>::SIZE, 4>(p, lift);
   if constexpr (D >= 3) lift_axis<Blk<D>::SIZE, 16>(p, lift);
+  if constexpr (D >= 4) lift_axis<Blk<D>::SIZE, 64>(p, lift);
 }
 template <int D, class I, class L> ZX_HD void inv_xform(I (&p)[Blk<D>::SIZE], L lift)
 {
+  if constexpr (D >= 4) lift_axis<Blk<D>::SIZE, 64>(p, lift);
   if constexpr (D >= 3) lift_axis<Blk<D>::SIZE, 16>(p, lift);
   if constexpr (D >= 2) lift_axis<Blk<D>::SIZE, 4>(p, lift);
   lift_axis<Blk<D>::SIZE, 1>(p, lift);
@@ -831,10 +848,146 @@ ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, 
 // built in registers and appended once, cut to the budget.  Planes are
 // transposed 16 at a time (a chunk), and a chunk no lane of the wave codes
 // is skipped.  Returns bits written.
+// 4-D blocks: the same coder over planes of 256 coefficients (bitplane.rs:
+// 161-218 keeps such a plane in four words), written as C's encode_ints and
+// decode_ints are, one plane and one group test at a time.  A 4-D block's
+// values do not fit in registers, so these loops run over arrays the compiler
+// keeps in scratch memory; 4-D fields are coded for completeness, not speed.
+ZX_HD uint32_t next_one256(const uint64_t (&x)[4], uint32_t n)
+{
+  for (uint32_t c = n >> 6; c < 4; c++) {
+    const uint64_t v = c == (n >> 6) ? x[c] >> (n & 63) << (n & 63) : x[c];
+    if (v) return 64 * c + ctz64(v);
+  }
+  return 256;
+}
+
+template <int P, class Sink>
+ZX_HD uint32_t encode_ints256(Sink &w, uint32_t maxbits, uint32_t maxprec,
+                              const typename Ints<P>::U (&u)[256])
+{
+  typedef typename Ints<P>::U U;
+  const uint32_t budget = with_maxbits(maxbits, maxprec, 256) ? maxbits : 0xffffffffu;
+  const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
+  U acc = 0;
+  for (int i = 0; i < 256; i++) acc |= u[i];
+  const uint32_t top = P == 32 ? 32u - clz32((uint32_t)acc) : 64u - clz64((uint64_t)acc);
+  uint32_t bits = budget;
+  uint32_t empty = (uint32_t)P - (top > kmin ? top : kmin);
+  if (empty > bits) empty = bits;
+  w.zeros(empty);
+  bits -= empty;
+  uint32_t k = (uint32_t)P - empty, n = 0;
+  while (bits != 0 && k > kmin) {
+    k--;
+    uint64_t x[4];
+    for (int c = 0; c < 4; c++) {
+      uint64_t v = 0;
+      for (int i = 0; i < 64; i++) v |= (uint64_t)((u[64 * c + i] >> k) & 1u) << i;
+      x[c] = v;
+    }
+    // the n coefficients already significant, verbatim
+    uint32_t m = umin(n, bits);
+    bits -= m;
+    for (int c = 0; m != 0; c++) {
+      const uint32_t t = umin(m, 64);
+      w.put128(x[c] & lowmask(t), 0ull, t);
+      m -= t;
+    }
+    // group tests: 1, the zeros before the next one-bit, and its 1 (implied
+    // for the last coefficient); 0 when no one-bit is left
+    while (bits != 0 && n < 256) {
+      const uint32_t q = next_one256(x, n);
+      if (q == 256) {
+        w.zeros(1);
+        bits -= 1;
+        break;
+      }
+      const uint32_t zeros = q - n;
+      const bool last = q == 255;
+      const uint32_t len = zeros + 2 - (last ? 1u : 0u);
+      const uint32_t take = umin(len, bits);
+      w.put128(1ull, 0ull, 1);
+      w.zeros(umin(zeros, take - 1));
+      if (take == len && !last) w.put128(1ull, 0ull, 1);
+      bits -= take;
+      n = q + 1;
+    }
+  }
+  return budget - bits;
+}
+
+template <int P, class Src>
+ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool round_last,
+                              typename Ints<P>::U (&u)[256], bool &skip)
+{
+  typedef typename Ints<P>::U U;
+  const bool constrained = with_maxbits(maxbits, maxprec, 256);
+  const uint32_t budget = constrained ? maxbits : 0xffffffffu;
+  const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
+  for (int i = 0; i < 256; i++) u[i] = 0;
+  uint32_t bits = budget, n = 0, m = 0, k = P;
+  bool act = true;
+  for (int plane = P - 1; plane >= 0; plane--) {
+    if (bits == 0) {
+      act = false;
+      break;
+    }
+    k -= 1;
+    m = 0;
+    if (k < kmin) {
+      act = false;
+      break;
+    }
+    m = umin(n, bits);
+    bits -= m;
+    uint64_t x[4] = {0, 0, 0, 0};
+    for (uint32_t c = 0, left = m; left != 0; c++) {
+      const uint32_t t = umin(left, 64);
+      x[c] = r.read(t);
+      left -= t;
+    }
+    const bool tested = bits != 0 && n < 256;
+    bool one = false;
+    while (bits != 0 && n < 256) {
+      bits -= 1;
+      one = r.read(1) != 0;
+      if (!one) break;
+      while (n < 255 && bits != 0) {
+        bits -= 1;
+        if (r.read(1)) break;
+        n++;
+      }
+      x[n >> 6] |= 1ull << (n & 63);
+      n++;
+    }
+    if (tested) m = one ? n : 256u;
+    for (int i = 0; i < 256; i++) u[i] |= (U)((x[i >> 6] >> (i & 63)) & 1u) << k;
+  }
+  if (act && bits != 0) {  // ran through plane 0 with budget left (kmin == 0)
+    m = 0;
+    k -= 1;
+  }
+  if (round_last) {
+    const uint32_t rm = constrained ? m : 0u;
+    const uint32_t rp = constrained ? (uint32_t)P - k : (uint32_t)P - (kmin - 1u);
+    if (rp < (uint32_t)P - 1) {
+      const U hi = (U)((Ints<P>::NB >> 2) >> rp);
+      const U lo = (U)((Ints<P>::NB >> 1) >> rp);
+      for (int i = 0; i < 256; i++) u[i] += (uint32_t)i < rm ? hi : lo;
+    }
+  }
+  skip = n == 0 && !round_last;
+  return budget - bits;
+}
+
 template <int P, int SIZE, class Sink>
 ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
                            const typename Ints<P>::U (&u)[SIZE])
 {
+  if constexpr (SIZE == 256) {
+    return encode_ints256<P>(w, maxbits, maxprec, u);
+  } else {
   typedef typename Ints<P>::U U;
   typedef typename PWord<SIZE>::T PW;
   const uint32_t budget = with_maxbits(maxbits, maxprec, SIZE) ? maxbits : 0xffffffffu;
@@ -880,6 +1033,7 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
     }
   }
   return budget - bits;
+  }
 }
 
 // decode_ints + decode_planes + round_state (bitplane.rs:626-770), written one
@@ -890,6 +1044,9 @@ template <int P, int SIZE, class Src>
 ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool round_last,
                            typename Ints<P>::U (&u)[SIZE], bool &skip)
 {
+  if constexpr (SIZE == 256) {
+    return decode_ints256<P>(r, maxbits, maxprec, round_last, u, skip);
+  } else {
   typedef typename Ints<P>::U U;
   typedef typename PWord<SIZE>::T PW;
   const bool constrained = with_maxbits(maxbits, maxprec, SIZE);
@@ -967,6 +1124,7 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
   }
   skip = zero && !round_last;
   return budget - bits;
+  }
 }
 
 // ---------------------------------------------------------------- integer blocks

@@ -50,23 +50,24 @@ ZX_HD uint32_t divide(uint32_t n, const Divisor &v)
 
 struct FieldDesc {
   void *data;          // element [0,0,0]; negative strides reach below it
-  int64_t n[3];        // extents (1 for unused axes)
-  int64_t s[3];        // element strides
-  uint32_t nb[3];      // blocks per axis
+  int64_t n[4];        // extents (1 for unused axes)
+  int64_t s[4];        // element strides
+  uint32_t nb[4];      // blocks per axis
   uint64_t nblocks;
-  Divisor bx, by;      // nb[0], nb[1] as divisors (set_block_grid)
+  Divisor bx, by, bz;  // nb[0], nb[1], nb[2] as divisors (set_block_grid)
 };
 
 // Sets the block grid of a field whose extents are set.
 ZX_HD void set_block_grid(FieldDesc &f)
 {
   f.nblocks = 1;
-  for (int a = 0; a < 3; a++) {
+  for (int a = 0; a < 4; a++) {
     f.nb[a] = (uint32_t)((f.n[a] + 3) / 4);
     f.nblocks *= f.nb[a];
   }
   f.bx = make_divisor(f.nb[0]);
   f.by = make_divisor(f.nb[1]);
+  f.bz = make_divisor(f.nb[2]);
 }
 
 ZX_HD uint32_t pad_src(uint32_t i, uint32_t n) { return i < n ? i : (i == 3 ? 0u : n - 1); }
@@ -83,7 +84,7 @@ ZX_HD void row_fence()
 
 struct BlockPos {
   int64_t origin;      // element offset of the block origin from data
-  uint32_t len[3];     // values inside the field along each axis
+  uint32_t len[4];     // values inside the field along each axis
   bool full;
 };
 
@@ -91,23 +92,30 @@ template <int D>
 ZX_HD BlockPos block_pos(const FieldDesc &f, uint64_t b)
 {
   BlockPos p;
-  uint32_t c[3] = {0, 0, 0};
+  uint32_t c[4] = {0, 0, 0, 0};
   const uint32_t bb = (uint32_t)b;
   if constexpr (D == 1) {
     c[0] = bb;
   } else if constexpr (D == 2) {
     c[1] = divide(bb, f.bx);
     c[0] = bb - c[1] * f.nb[0];
-  } else {
+  } else if constexpr (D == 3) {
     const uint32_t t = divide(bb, f.bx);
     c[0] = bb - t * f.nb[0];
     c[2] = divide(t, f.by);
     c[1] = t - c[2] * f.nb[1];
+  } else {
+    const uint32_t t = divide(bb, f.bx);
+    c[0] = bb - t * f.nb[0];
+    const uint32_t u = divide(t, f.by);
+    c[1] = t - u * f.nb[1];
+    c[3] = divide(u, f.bz);
+    c[2] = u - c[3] * f.nb[2];
   }
   p.origin = 0;
   p.full = true;
   ZX_UNROLL
-  for (int a = 0; a < 3; a++) {
+  for (int a = 0; a < 4; a++) {
     if (a < D) {
       const int64_t o = 4 * (int64_t)c[a];
       const int64_t rem = f.n[a] - o;
@@ -134,20 +142,21 @@ ZX_HD void gather(const FieldDesc &f, const BlockPos &p,
       int64_t off = (int64_t)(i & 3) * f.s[0];
       if (D >= 2) off += (int64_t)((i >> 2) & 3) * f.s[1];
       if (D >= 3) off += (int64_t)((i >> 4) & 3) * f.s[2];
+      if (D >= 4) off += (int64_t)((i >> 6) & 3) * f.s[3];
       v[i] = base[off];
     }
   } else {
     // per-axis offsets of the padded positions, then one row of four at a
     // time (see row_fence)
-    int64_t o[3][4];
+    int64_t o[4][4];
     ZX_UNROLL
-    for (int a = 0; a < 3; a++) {
+    for (int a = 0; a < 4; a++) {
       ZX_UNROLL
       for (int k = 0; k < 4; k++) o[a][k] = a < D ? (int64_t)pad_src((uint32_t)k, p.len[a]) * f.s[a] : 0;
     }
     ZX_UNROLL
     for (int r = 0; r < Blk<D>::SIZE / 4; r++) {
-      const U *row = base + o[1][r & 3] + o[2][(r >> 2) & 3];
+      const U *row = base + o[1][r & 3] + o[2][(r >> 2) & 3] + (D >= 4 ? o[3][(r >> 4) & 3] : 0);
       ZX_UNROLL
       for (int x = 0; x < 4; x++) v[4 * r + x] = row[o[0][x]];
       row_fence();
@@ -164,9 +173,11 @@ ZX_HD void scatter(const FieldDesc &f, const BlockPos &p,
   U *base = (U *)f.data + p.origin;
   ZX_UNROLL
   for (int r = 0; r < Blk<D>::SIZE / 4; r++) {
-    const uint32_t y = r & 3, z = (r >> 2) & 3;
-    const bool row_in = p.full || ((D < 2 || y < p.len[1]) && (D < 3 || z < p.len[2]));
-    U *row = base + (D >= 2 ? (int64_t)y * f.s[1] : 0) + (D >= 3 ? (int64_t)z * f.s[2] : 0);
+    const uint32_t y = r & 3, z = (r >> 2) & 3, w = (r >> 4) & 3;
+    const bool row_in =
+        p.full || ((D < 2 || y < p.len[1]) && (D < 3 || z < p.len[2]) && (D < 4 || w < p.len[3]));
+    U *row = base + (D >= 2 ? (int64_t)y * f.s[1] : 0) + (D >= 3 ? (int64_t)z * f.s[2] : 0) +
+             (D >= 4 ? (int64_t)w * f.s[3] : 0);
     ZX_UNROLL
     for (int x = 0; x < 4; x++)
       if (row_in && (p.full || (uint32_t)x < p.len[0])) row[(int64_t)x * f.s[0]] = v[4 * r + x];

@@ -1,5 +1,5 @@
 // One instantiation of the field codec's kernels (zfpx_kernels.inc.h), picked
-// at compile time so that the 24 (scalar type, dimensionality, reversible)
+// at compile time so that the 32 (scalar type, dimensionality, reversible)
 // variants build in parallel: ZX_S is the scalar type, ZX_TAG its short name,
 // ZX_D the dimensionality and ZX_REV 1 for reversible mode.
 #include "zfpx_kernels.inc.h"

@@ -68,7 +68,8 @@ __device__ __forceinline__ void gather_rows(const FieldDesc &f, const BlockPos &
   for (int r = 0; r < Blk<D>::SIZE / 4; r++) {
     int64_t off = 0;
     if (D >= 2) off += (int64_t)(r & 3) * f.s[1];
-    if (D >= 3) off += (int64_t)(r >> 2) * f.s[2];
+    if (D >= 3) off += (int64_t)((r >> 2) & 3) * f.s[2];
+    if (D >= 4) off += (int64_t)(r >> 4) * f.s[3];
     if constexpr (sizeof(U) == 4) {
       const uint4 q = *(const uint4 *)(base + off);
       v[4 * r + 0] = q.x; v[4 * r + 1] = q.y; v[4 * r + 2] = q.z; v[4 * r + 3] = q.w;
@@ -94,7 +95,8 @@ __device__ __forceinline__ void scatter_rows(const FieldDesc &f, const BlockPos 
   for (int r = 0; r < Blk<D>::SIZE / 4; r++) {
     int64_t off = 0;
     if (D >= 2) off += (int64_t)(r & 3) * f.s[1];
-    if (D >= 3) off += (int64_t)(r >> 2) * f.s[2];
+    if (D >= 3) off += (int64_t)((r >> 2) & 3) * f.s[2];
+    if (D >= 4) off += (int64_t)(r >> 4) * f.s[3];
     if constexpr (sizeof(U) == 4) {
       *(uint4 *)(base + off) = make_uint4(v[4 * r], v[4 * r + 1], v[4 * r + 2], v[4 * r + 3]);
     } else {
@@ -373,29 +375,41 @@ __global__ __launch_bounds__(64) void k_batch_walk(BatchArgs a)
 
 // ------------------------------------------------------------------ launchers
 
+// 4-D blocks (256 values) keep to the global-memory kernels and have no batch
+// launches.
 template <class S, int D, bool REV>
-static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
+static hipError_t launch_global_encode_t(const EncodeArgs &a, hipStream_t s)
 {
-  if (a.field.nblocks == 0) return hipSuccess;
-  FxPlan p;
-  if (!a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p)) {
-    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
-    const size_t bytes = (size_t)p.lds_words * 8;
-    if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
-    else hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
-    return hipGetLastError();
-  }
-  if (!a.lengths && a.offsets && a.vbits) {
-    vx_plan(a.vbits, a.field.nblocks, a.slice, a.vcap, p);
-    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
-    hipLaunchKernelGGL((k_encode_vx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-    hipLaunchKernelGGL((k_encode_dense<S, D, REV>), dim3((grid + p.T - 1) / p.T), dim3(p.T), 0, s, a, p, grid);
-    return hipGetLastError();
-  }
   const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
   if (a.lengths) hipLaunchKernelGGL((k_encode<S, D, REV, true>), dim3(grid), dim3(kThreads), 0, s, a);
   else hipLaunchKernelGGL((k_encode<S, D, REV, false>), dim3(grid), dim3(kThreads), 0, s, a);
   return hipGetLastError();
+}
+
+template <class S, int D, bool REV>
+static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
+{
+  if (a.field.nblocks == 0) return hipSuccess;
+  if constexpr (D == 4) {
+    return launch_global_encode_t<S, D, REV>(a, s);
+  } else {
+    FxPlan p;
+    if (!a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p)) {
+      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+      const size_t bytes = (size_t)p.lds_words * 8;
+      if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
+      else hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
+      return hipGetLastError();
+    }
+    if (!a.lengths && a.offsets && a.vbits) {
+      vx_plan(a.vbits, a.field.nblocks, a.slice, a.vcap, p);
+      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+      hipLaunchKernelGGL((k_encode_vx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+      hipLaunchKernelGGL((k_encode_dense<S, D, REV>), dim3((grid + p.T - 1) / p.T), dim3(p.T), 0, s, a, p, grid);
+      return hipGetLastError();
+    }
+    return launch_global_encode_t<S, D, REV>(a, s);
+  }
 }
 
 template <class S, int D, bool REV>
@@ -406,12 +420,14 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
     return hipGetLastError();
   }
   if (a.field.nblocks == 0) return hipSuccess;
-  FxPlan p;
-  if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p)) {
-    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
-    if (p.flat) hipLaunchKernelGGL((k_decode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-    else hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-    return hipGetLastError();
+  if constexpr (D != 4) {
+    FxPlan p;
+    if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p)) {
+      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+      if (p.flat) hipLaunchKernelGGL((k_decode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+      else hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+      return hipGetLastError();
+    }
   }
   const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
   hipLaunchKernelGGL((k_decode<S, D, REV>), dim3(grid), dim3(kThreads), 0, s, a);
@@ -421,27 +437,35 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
 template <class S, int D, bool REV>
 static hipError_t launch_batch_encode_t(const BatchArgs &a, bool count, hipStream_t s)
 {
-  if (a.nblocks == 0) return hipSuccess;
-  const uint32_t T = a.threads ? a.threads : kThreads;
-  const uint64_t grid = (a.nblocks + T - 1) / T;
-  if (count) hipLaunchKernelGGL((k_batch_encode<S, D, REV, true>), dim3(grid), dim3(T), 0, s, a);
-  else hipLaunchKernelGGL((k_batch_encode<S, D, REV, false>), dim3(grid), dim3(T), 0, s, a);
-  return hipGetLastError();
+  if constexpr (D == 4) {
+    return hipErrorNotSupported;
+  } else {
+    if (a.nblocks == 0) return hipSuccess;
+    const uint32_t T = a.threads ? a.threads : kThreads;
+    const uint64_t grid = (a.nblocks + T - 1) / T;
+    if (count) hipLaunchKernelGGL((k_batch_encode<S, D, REV, true>), dim3(grid), dim3(T), 0, s, a);
+    else hipLaunchKernelGGL((k_batch_encode<S, D, REV, false>), dim3(grid), dim3(T), 0, s, a);
+    return hipGetLastError();
+  }
 }
 
 template <class S, int D, bool REV>
 static hipError_t launch_batch_decode_t(const BatchArgs &a, bool walk, hipStream_t s)
 {
-  if (walk) {
-    if (a.njobs == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_batch_walk<S, D, REV>), dim3((a.njobs + 63) / 64), dim3(64), 0, s, a);
+  if constexpr (D == 4) {
+    return hipErrorNotSupported;
+  } else {
+    if (walk) {
+      if (a.njobs == 0) return hipSuccess;
+      hipLaunchKernelGGL((k_batch_walk<S, D, REV>), dim3((a.njobs + 63) / 64), dim3(64), 0, s, a);
+      return hipGetLastError();
+    }
+    if (a.nblocks == 0) return hipSuccess;
+    const uint32_t T = a.threads ? a.threads : kThreads;
+    const uint64_t grid = (a.nblocks + T - 1) / T;
+    hipLaunchKernelGGL((k_batch_decode<S, D, REV>), dim3(grid), dim3(T), 0, s, a);
     return hipGetLastError();
   }
-  if (a.nblocks == 0) return hipSuccess;
-  const uint32_t T = a.threads ? a.threads : kThreads;
-  const uint64_t grid = (a.nblocks + T - 1) / T;
-  hipLaunchKernelGGL((k_batch_decode<S, D, REV>), dim3(grid), dim3(T), 0, s, a);
-  return hipGetLastError();
 }
 
 }  // namespace zx

@@ -258,8 +258,8 @@ def _describe(arr):
 def make_field(arr):
     """A zfp_field over an array; zfp's x is the array's last axis."""
     ztype, ptr, shape, strides = _describe(arr)
-    if not 1 <= len(shape) <= 3:
-        raise FieldError("1-D to 3-D arrays only")
+    if not 1 <= len(shape) <= 4:
+        raise FieldError("1-D to 4-D arrays only")
     f = zfp_field()
     f.type = ztype
     n = list(reversed(shape)) + [0] * (4 - len(shape))
