@@ -2,7 +2,7 @@
  * which zfp-rs-ffi exports without declaring them in its zfp.h.  Each codes one block at the
  * stream's cursor with the stream's parameters and returns the block's bits (0 for a null
  * block, a length outside 1..4 in a partial block, or a failure).  Blocks run on the GPU as
- * one-block fields; 4-D blocks are not built and return 0. */
+ * one-block fields. */
 #ifndef ZFP_BLOCK_H
 #define ZFP_BLOCK_H
 

@@ -1185,7 +1185,8 @@ bool grow_pinned(void *&p, size_t &cap, size_t need)
 }
 
 // Checks an array as zfp_compress / zfp_decompress do; false when the call
-// would fail before touching the stream.
+// would fail before touching the stream, and for 4-D arrays (the batch
+// kernels are built for 1-3-D).
 bool batch_plan(zfp_stream *z, const zfp_field *f, bool decode, Plan &p)
 {
   if (!z || !f || !z->stream || !f->data) return false;
