@@ -25,7 +25,9 @@ Also reported:
                 req/s, and with every request arriving at once;
   variable_rate the variable-rate configurations (384^3 f64 accuracy and
                 precision, 8192^2 f32 precision, 256^3 i32 reversible):
-                compress and decompress GB/s through the offset index.
+                compress and decompress GB/s through the offset index;
+  four_d        4-D fields (64^4 f32 rate 8 and precision 16, 32^4 f64
+                reversible): compress and decompress GB/s.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rate 8] [--n 512]
 """
@@ -426,6 +428,48 @@ def variable_rate_leg(steps=5):
     v3 = torch.randint(-3, 4, (256 ** 3,), generator=g, device="cuda", dtype=torch.int32)
     v3 = torch.cumsum(v3, 0, dtype=torch.int32).reshape(256, 256, 256)
     cases.append(("256^3 i32 random walk reversible", v3, Z.ZfpConfig.reversible(), 0))
+    out = _time_cases(cases, steps, note)
+    del v1, v2, v3, cases
+    torch.cuda.empty_cache()
+    return out
+
+
+def four_d_leg(steps=3):
+    """4-D fields: 64^4 f32 (a smooth separable field plus 1e-3 noise) at fixed
+    rate 8 and fixed precision 16, and 32^4 f64 reversible; compress and
+    decompress GB/s as for the variable-rate configurations.  4-D blocks run
+    the global-memory kernels with their 256 values in scratch memory."""
+    import torch
+
+    import zfp_hip as Z
+
+    def note(msg):
+        print(f"[four_d] {msg}", file=sys.stderr, flush=True)
+
+    t = torch.linspace(0.0, 6.283185307179586, 64, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(44)
+    v = (torch.sin(t)[None, None, None, :] * torch.cos(2 * t)[None, None, :, None]
+         + torch.sin(3 * t)[None, :, None, None] * torch.cos(t)[:, None, None, None])
+    v = (v + 1e-3 * torch.randn(v.shape, generator=g, device="cuda")).contiguous()
+    w = v[:32, :32, :32, :32].to(torch.float64).contiguous()
+    cases = [("64^4 f32 fixed-rate 8", v, Z.ZfpConfig.fixed_rate(8.0, Z.TYPE_FLOAT, 4), None),
+             ("64^4 f32 fixed-precision 16", v, Z.ZfpConfig.fixed_precision(16), None),
+             ("32^4 f64 reversible", w, Z.ZfpConfig.reversible(), 0)]
+    out = _time_cases(cases, steps, note)
+    del v, w, cases
+    torch.cuda.empty_cache()
+    return out
+
+
+def _time_cases(cases, steps, note):
+    """Best compress and decompress time of `steps` (after one untimed step)
+    for each (name, device tensor, config, tolerance) case; tolerance 0 checks
+    an exact round trip, a float checks the largest error."""
+    import torch
+
+    import zfp_hip as Z
+
     out = []
     for name, x, cfg, tol in cases:
         cap = Z.maximum_size(cfg, x)
@@ -459,8 +503,6 @@ def variable_rate_leg(steps=5):
         note(str(r))
         out.append(r)
         del buf, y
-    del v1, v2, v3, cases
-    torch.cuda.empty_cache()
     return out
 
 
@@ -629,6 +671,7 @@ def main():
         result["serving"] = serving_leg(args.serving_requests, args.serving_rate)
     if not args.no_variable_rate:
         result["variable_rate"] = variable_rate_leg()
+        result["four_d"] = four_d_leg()
 
     if not args.no_cpu_baseline:
         def gpu_words(slab):
