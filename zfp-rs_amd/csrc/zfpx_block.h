@@ -953,16 +953,29 @@ ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool r
       bits -= 1;
       one = r.read(1) != 0;
       if (!one) break;
-      while (n < 255 && bits != 0) {
-        bits -= 1;
-        if (r.read(1)) break;
-        n++;
+      // the zeros before the next one-bit (read up to 64 at a time), up to
+      // the last coefficient, whose one-bit is implied, or the budget
+      for (uint32_t left = umin(255 - n, bits); left != 0;) {
+        const uint32_t span = umin(left, 64);
+        const uint32_t z = ctz64(r.peek());
+        if (z < span) {
+          r.consume(z + 1);
+          bits -= z + 1;
+          n += z;
+          break;
+        }
+        r.consume(span);
+        bits -= span;
+        n += span;
+        left -= span;
       }
       x[n >> 6] |= 1ull << (n & 63);
       n++;
     }
     if (tested) m = one ? n : 256u;
-    for (int i = 0; i < 256; i++) u[i] |= (U)((x[i >> 6] >> (i & 63)) & 1u) << k;
+    // fold the plane's one-bits in: only those coefficients change
+    for (int c = 0; c < 4; c++)
+      for (uint64_t b = x[c]; b != 0; b &= b - 1) u[64 * c + ctz64(b)] |= (U)1 << k;
   }
   if (act && bits != 0) {  // ran through plane 0 with budget left (kmin == 0)
     m = 0;
