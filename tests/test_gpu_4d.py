@@ -154,3 +154,23 @@ def test_4d_block_entry_points(ztype):
         O.decode_block(st, oc, ztype, 4, ref_big.reshape(-1), 1 + 5 + 25 + 125, strides4, list(n))
         assert np.array_equal(dst_big.view(u), ref_big.view(u))
         s.close()
+
+
+def test_batch_codes_4d_arrays_like_single_calls():
+    """A batch holding 4-D arrays codes them (with the single-array call) and
+    gives every stream the bytes and size of zfp_compress."""
+    a3 = torch.from_numpy(field(O.FLOAT, (9, 10, 11), 3)).cuda()
+    a4 = torch.from_numpy(field(O.FLOAT, (5, 3, 6, 9), 4)).cuda()
+    cfg = Z.ZfpConfig.fixed_precision(18)
+    arrays = [a3, a4, a3, a4]
+    bufs = [torch.zeros(65536, dtype=torch.int64, device="cuda") for _ in arrays]
+    streams = [Z.Stream(b, cfg) for b in bufs]
+    _, sizes = Z._batch(Z.lib().zfp_hip_compress_batch, streams, arrays, Z.BATCH_DEVICE_FIELDS)
+    for a, b, n in zip(arrays, bufs, sizes):
+        ref = torch.zeros(65536, dtype=torch.int64, device="cuda")
+        s = Z.Stream(ref, cfg)
+        assert n > 0 and s.compress(a) == n
+        s.close()
+        assert torch.equal(b, ref)
+    for s in streams:
+        s.close()

@@ -1185,13 +1185,12 @@ bool grow_pinned(void *&p, size_t &cap, size_t need)
 }
 
 // Checks an array as zfp_compress / zfp_decompress do; false when the call
-// would fail before touching the stream, and for 4-D arrays (the batch
-// kernels are built for 1-3-D).
+// would fail before touching the stream.
 bool batch_plan(zfp_stream *z, const zfp_field *f, bool decode, Plan &p)
 {
   if (!z || !f || !z->stream || !f->data) return false;
   if (decode && !valid_type(f->type)) return false;
-  if (!make_plan(f, p) || p.dims > 3) return false;
+  if (!make_plan(f, p)) return false;
   if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return false;
   return true;
 }
@@ -1239,7 +1238,9 @@ void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, siz
       continue;
     }
     s->batch_epoch = epoch;
-    if (p.nblocks == 0) {  // nothing to code: the calls' own handling
+    // nothing to code (the calls' own handling), or 4-D blocks, which the
+    // batch kernels are not built for; the stream's later arrays follow
+    if (p.nblocks == 0 || p.dims > 3) {
       S.later.push_back(i);
       continue;
     }
