@@ -111,6 +111,15 @@ size_t zfp_hip_batch_wait(zfp_hip_batch *submission, size_t *sizes);
 /* zfp_true when a submission's device work has completed (wait will not block). */
 zfp_bool zfp_hip_batch_ready(const zfp_hip_batch *submission);
 
+/* zfp_true when two index tuples of the field may address one element
+ * (FieldPlan::strides_may_alias, src/field_plan.rs:92-118): the axes of two
+ * or more values, sorted by |stride|, do not each step past the reach of the
+ * smaller ones.  zfp_decompress into such a field leaves every element the
+ * value the reference's serial decode writes last (src/decompress.rs:157-163):
+ * the blocks are decoded into a dense device copy, then stored in that order.
+ * zfp_false for a null or invalid field. */
+zfp_bool zfp_hip_field_may_alias(const zfp_field *field);
+
 /* Device the library runs on (the current HIP device when first used). */
 int zfp_hip_device(void);
 

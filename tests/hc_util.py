@@ -33,6 +33,9 @@ def lib():
                                     P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
                                     ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_uint32]
         L.hc_decompress.restype = ctypes.c_uint64
+        L.hc_alias_scatter.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_int64),
+                                       P(ctypes.c_int64), ctypes.c_int64, ctypes.c_int, ctypes.c_uint32]
+        L.hc_alias_scatter.restype = ctypes.c_int
         _lib = L
     return _lib
 

@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <vector>
 
+#include "zfpx_alias.h"
 #include "zfpx_fx.h"
 
 using namespace zx;
@@ -256,6 +257,46 @@ uint64_t hc_decompress(int type, int dims, int rev, void *data, const int64_t *n
   case 8: return decomp_d<double, false>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
   default: return decomp_d<double, true>(dims, f, c, words, nwords, start_bit, fixed_bits, lengths, lds_T);
   }
+}
+
+// The alias scatter of zfpx_alias.hip run on the CPU, its two passes in the
+// opposite thread order to the natural one (the result must not depend on it).
+int hc_alias_scatter(const void *dense, void *lowest, const int64_t *n, const int64_t *s,
+                     int64_t imin, int dims, uint32_t esz)
+{
+  AliasArgs a{};
+  a.dense = dense;
+  a.lowest = lowest;
+  a.count = 1;
+  int64_t lo = 0, hi = 0;
+  for (int k = 0; k < 4; k++) {
+    a.n[k] = k < dims ? n[k] : 1;
+    a.s[k] = k < dims ? s[k] : 0;
+    if (k < dims) {
+      a.count *= (uint64_t)n[k];
+      const int64_t e = s[k] * (n[k] - 1);
+      if (e < 0) lo += e;
+      else hi += e;
+    }
+  }
+  a.imin = imin;
+  a.span = (uint64_t)(hi - lo + 1);
+  a.dims = dims;
+  a.esz = esz;
+  if (lo != imin) return -1;
+  std::vector<uint64_t> best(a.span, 0);
+  for (uint64_t i = a.count; i-- > 0;) {
+    const AliasSlot q = alias_slot(a, i);
+    if ((uint64_t)q.addr >= a.span) return -2;
+    if (q.ticket > best[q.addr]) best[q.addr] = q.ticket;
+  }
+  for (uint64_t i = a.count; i-- > 0;) {
+    const AliasSlot q = alias_slot(a, i);
+    if (best[q.addr] != q.ticket) continue;
+    if (esz == 4) ((uint32_t *)lowest)[q.addr] = ((const uint32_t *)dense)[i];
+    else ((uint64_t *)lowest)[q.addr] = ((const uint64_t *)dense)[i];
+  }
+  return 0;
 }
 
 // n / d through the block grid's invariant-divisor path, for count values

@@ -58,6 +58,8 @@ struct Workspace {
   void *lens = nullptr; size_t lens_cap = 0;
   void *offs = nullptr; size_t offs_cap = 0;
   void *tiles = nullptr; size_t tiles_cap = 0;
+  void *dense = nullptr; size_t dense_cap = 0;  // decode of an aliasing field
+  void *best = nullptr; size_t best_cap = 0;    // its per-element tickets
   void *one = nullptr;  // one pinned host word for reading back a scan total
   int device = -1;
 
@@ -626,12 +628,48 @@ struct Plan {
   int64_t s[4];
   int64_t imin, imax;  // element span relative to element [0,0,0,0]
   size_t esz;
-  uint64_t nblocks;
+  uint64_t nblocks;    // saturates at UINT64_MAX, as the reference's num_blocks
+  uint64_t nelems;     // saturates at UINT64_MAX, as num_elements
+  bool fits;           // logical_shape_fits (field.rs:566-569)
+  bool may_alias;      // strides_may_alias (field_plan.rs:92-118)
   uint32_t nb[3];
 };
 
-// checked_index_span + effective strides (field.rs:580-647), block grid
-// (field_plan.rs:41-77); false where the reference rejects the field.
+// Whether two index tuples of the field may address one element: the active
+// axes of two or more values, sorted by |stride|, must each step past the
+// reach of the smaller ones (field_plan.rs:92-118; true means "cannot be
+// ruled out").
+bool strides_may_alias(const Plan &p)
+{
+  uint64_t st[4], dm[4];
+  int k = 0;
+  for (int a = 0; a < p.dims; a++) {
+    if (p.n[a] >= 2) {
+      st[k] = p.s[a] < 0 ? 0 - (uint64_t)p.s[a] : (uint64_t)p.s[a];
+      dm[k] = (uint64_t)p.n[a];
+      k++;
+    }
+  }
+  for (int i = 1; i < k; i++)  // insertion sort by (stride, extent)
+    for (int j = i; j > 0 && (st[j] < st[j - 1] || (st[j] == st[j - 1] && dm[j] < dm[j - 1])); j--) {
+      std::swap(st[j], st[j - 1]);
+      std::swap(dm[j], dm[j - 1]);
+    }
+  uint64_t reach = 1;
+  for (int i = 0; i < k; i++) {
+    if (st[i] < reach) return true;
+    const unsigned __int128 r = (unsigned __int128)st[i] * (dm[i] - 1) + reach;
+    reach = r > UINT64_MAX ? UINT64_MAX : (uint64_t)r;
+  }
+  return false;
+}
+
+// The field as the reference's FFI takes it (field_to_rust, zfp-rs-ffi/src/
+// field.rs:43-86): checked_index_span with natural strides for zero entries
+// (field.rs:621-647) and a span of at most isize::MAX bytes; false where that
+// conversion fails.  The block grid (field_plan.rs:41-77), the element and
+// block counts (saturating), logical_shape_fits and strides_may_alias come
+// with it.
 bool make_plan(const zfp_field *f, Plan &p)
 {
   if (!f || !valid_type(f->type)) return false;
@@ -640,38 +678,60 @@ bool make_plan(const zfp_field *f, Plan &p)
   p.dims = dimensionality(f);
   const size_t dims[4] = {f->nx, f->ny, f->nz, f->nw};
   const ptrdiff_t strides[4] = {f->sx, f->sy, f->sz, f->sw};
-  int64_t below = 1;
-  bool below_ok = true;
-  p.imin = p.imax = 0;
-  p.nblocks = 1;
+  __int128 imin = 0, imax = 0;
+  unsigned __int128 below = 1, elems = 1, blocks = 1;  // saturate at 2^64
+  const unsigned __int128 kSat = (unsigned __int128)UINT64_MAX + 1;
   for (int a = 0; a < 4; a++) {
     p.n[a] = 1;
     p.s[a] = 0;
   }
   for (int a = 0; a < p.dims; a++) {
-    const int64_t d = (int64_t)dims[a];
-    if (d < 0) return false;
+    const uint64_t d = (uint64_t)dims[a];
+    const unsigned __int128 natural = below;
+    below = below * d < kSat ? below * d : kSat;
+    elems = elems * d < kSat ? elems * d : kSat;
+    blocks = blocks * ((d + 3) / 4) < kSat ? blocks * ((d + 3) / 4) : kSat;
     int64_t st = strides[a];
+    if (d == 0) {
+      p.n[a] = 0;
+      p.s[a] = st != 0 ? st : (natural <= (unsigned __int128)INT64_MAX ? (int64_t)natural : INT64_MAX);
+      continue;
+    }
+    if (d > (uint64_t)INT64_MAX) return false;
     if (st == 0) {
-      if (!below_ok) return false;
-      st = below;
+      if (natural > (unsigned __int128)INT64_MAX) return false;
+      st = (int64_t)natural;
     }
-    p.n[a] = d;
+    p.n[a] = (int64_t)d;
     p.s[a] = st;
-    if (d > 0) {
-      __int128 ext = (__int128)st * (d - 1);
-      if (ext > INT64_MAX / 2 || ext < INT64_MIN / 2) return false;
-      if (ext < 0) p.imin += (int64_t)ext;
-      else p.imax += (int64_t)ext;
-    }
-    __int128 nbl = (__int128)below * d;
-    if (nbl > INT64_MAX) below_ok = false;
-    else below = (int64_t)nbl;
-    p.nblocks *= (uint64_t)((d + 3) / 4);
+    const __int128 ext = (__int128)st * (__int128)(d - 1);
+    if (ext > INT64_MAX || ext < INT64_MIN) return false;
+    if (ext < 0) imin += ext;
+    else imax += ext;
+    if (imin < INT64_MIN || imax > INT64_MAX) return false;
   }
+  const __int128 span = imax - imin + 1;
+  if (span > INT64_MAX || span * (__int128)p.esz > INT64_MAX) return false;
+  p.imin = (int64_t)imin;
+  p.imax = (int64_t)imax;
+  p.nelems = elems >= kSat ? UINT64_MAX : (uint64_t)elems;
+  p.nblocks = blocks >= kSat ? UINT64_MAX : (uint64_t)blocks;
+  p.fits = elems <= (unsigned __int128)INT64_MAX && blocks < kSat;
   for (int a = 0; a < 3; a++) p.nb[a] = (uint32_t)((p.n[a] + 3) / 4);
-  if (p.nblocks > 0xffffffffull) return false;
+  p.may_alias = strides_may_alias(p);
   return true;
+}
+
+// Blocks the codec's kernels can index: their grid arithmetic is 32-bit.
+constexpr uint64_t kMaxCodecBlocks = 0xffffffffull;
+
+// make_plan for zfp_compress / zfp_decompress and the batched calls: the
+// reference also rejects a shape that does not fit (validate_layout,
+// field.rs:84-106, ShapeTooLarge), and this library a field of more than
+// kMaxCodecBlocks blocks.
+bool codec_plan(const zfp_field *f, Plan &p)
+{
+  return make_plan(f, p) && p.fits && p.nblocks <= kMaxCodecBlocks;
 }
 
 bool contiguous(const Plan &p)
@@ -894,11 +954,27 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   bool staged = false;
   Timer tm(xp, st);
   if (!stage_field(W, f, p, !contiguous(p), st, dorigin, staged)) return false;
+  // A field whose strides may alias is decoded into a dense copy first; the
+  // alias scatter then leaves each element the serial decode's last value.
+  Plan q = p;
+  void *korigin = dorigin;
+  if (p.may_alias) {
+    int64_t nat = 1;
+    for (int a = 0; a < p.dims; a++) {
+      q.s[a] = nat;
+      nat *= p.n[a];
+    }
+    q.imin = 0;
+    q.imax = nat - 1;
+    if (!Workspace::grow(W.dense, W.dense_cap, (size_t)p.nelems * p.esz)) return false;
+    if (!Workspace::grow(W.best, W.best_cap, (size_t)(p.imax - p.imin + 1) * 8)) return false;
+    korigin = W.dense;
+  }
   tm.mark(0);
   zx::DecodeArgs a{};
-  a.field = field_desc(p, dorigin);
+  a.field = field_desc(q, korigin);
   a.cfg = c;
-  a.vec = vec_ok(p, dorigin);
+  a.vec = vec_ok(q, korigin);
   a.start = start - 64 * w0;
   a.offsets = nullptr;
   if (!Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(p.nblocks) * 8 + 8)) return false;
@@ -956,6 +1032,21 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   if (!whole) ZCHECK(zx_launch_fill(maxword, 1, 0, st));
   tm.mark(1);
   ZCHECK(launch_decode(p.type, p.dims, rev, a, nullptr, st));
+  if (p.may_alias) {
+    zx::AliasArgs aa{};
+    aa.dense = W.dense;
+    aa.lowest = (char *)dorigin + p.imin * (int64_t)p.esz;
+    for (int k = 0; k < 4; k++) {
+      aa.n[k] = p.n[k];
+      aa.s[k] = p.s[k];
+    }
+    aa.imin = p.imin;
+    aa.count = p.nelems;
+    aa.span = (uint64_t)(p.imax - p.imin + 1);
+    aa.dims = p.dims;
+    aa.esz = (uint32_t)p.esz;
+    ZCHECK(zx_launch_alias_scatter(aa, (unsigned long long *)W.best, st));
+  }
   tm.mark(2);
   if (!whole) ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 1, maxword, 8, hipMemcpyDeviceToHost, st));
   if (staged) {
@@ -1190,7 +1281,7 @@ bool batch_plan(zfp_stream *z, const zfp_field *f, bool decode, Plan &p)
 {
   if (!z || !f || !z->stream || !f->data) return false;
   if (decode && !valid_type(f->type)) return false;
-  if (!make_plan(f, p)) return false;
+  if (!codec_plan(f, p)) return false;
   if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return false;
   return true;
 }
@@ -1217,7 +1308,8 @@ struct BatchDev {
 };
 
 // Plans the arrays; sizes[] is zeroed.  Arrays the batch does not take
-// (invalid, host-resident, a stream met before, no blocks) go to `later`.
+// (host-resident, a stream met before, no blocks, 4-D, a decode into
+// aliasing strides) go to `later`; invalid arrays are skipped (size 0).
 void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_t count,
                    size_t *sizes, uint flags, BatchSet &S)
 {
@@ -1238,9 +1330,11 @@ void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, siz
       continue;
     }
     s->batch_epoch = epoch;
-    // nothing to code (the calls' own handling), or 4-D blocks, which the
-    // batch kernels are not built for; the stream's later arrays follow
-    if (p.nblocks == 0 || p.dims > 3) {
+    // nothing to code (the calls' own handling), 4-D blocks, which the batch
+    // kernels are not built for, or a decode into strides that may alias,
+    // which needs the single-array call's ordered scatter; the stream's later
+    // arrays follow
+    if (p.nblocks == 0 || p.dims > 3 || (!encode && p.may_alias)) {
       S.later.push_back(i);
       continue;
     }
@@ -2050,7 +2144,7 @@ size_t zfp_compress(zfp_stream *z, const zfp_field *f)
 {
   if (!z || !f || !z->stream || !f->data) return 0;
   Plan p;
-  if (!make_plan(f, p)) return 0;
+  if (!codec_plan(f, p)) return 0;
   if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return 0;
   bitstream *s = z->stream;
   const zx::Cfg c = cfg_of(z);
@@ -2069,7 +2163,7 @@ size_t zfp_decompress(zfp_stream *z, zfp_field *f)
 {
   if (!z || !f || !z->stream || !f->data || !valid_type(f->type)) return 0;
   Plan p;
-  if (!make_plan(f, p)) return 0;
+  if (!codec_plan(f, p)) return 0;
   if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return 0;
   bitstream *s = z->stream;
   const zx::Cfg c = cfg_of(z);
@@ -2332,9 +2426,7 @@ size_t zfp_field_size(const zfp_field *f, size_t *dims)
   }
   Plan p;
   if (!make_plan(f, p)) return 0;
-  size_t n = 1;
-  for (int a = 0; a < p.dims; a++) n *= (size_t)p.n[a];
-  return n;
+  return (size_t)p.nelems;
 }
 size_t zfp_field_size_bytes(const zfp_field *f)
 {
@@ -2352,6 +2444,11 @@ zfp_bool zfp_field_is_contiguous(const zfp_field *f)
   Plan p;
   if (!f || !make_plan(f, p)) return zfp_false;
   return contiguous(p) ? zfp_true : zfp_false;
+}
+zfp_bool zfp_hip_field_may_alias(const zfp_field *f)
+{
+  Plan p;
+  return f && make_plan(f, p) && p.may_alias ? zfp_true : zfp_false;
 }
 zfp_bool zfp_field_stride(const zfp_field *f, ptrdiff_t *strides)
 {

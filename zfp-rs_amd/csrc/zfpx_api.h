@@ -1,10 +1,11 @@
 // zfpx_api.h -- launch interface between the C ABI (zfp_capi.cpp) and the
-// kernels (zfpx_kernels_*.hip, zfpx_scan.hip).
+// kernels (zfpx_kernels_*.hip, zfpx_scan.hip, zfpx_alias.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "zfpx_alias.h"
 #include "zfpx_field.h"
 
 namespace zx {
@@ -70,6 +71,10 @@ struct BatchArgs {
 };
 
 }  // namespace zx
+
+// zfpx_alias.hip: ticket pass and store pass; `best` holds `span` words of
+// scratch.
+hipError_t zx_launch_alias_scatter(const zx::AliasArgs &a, unsigned long long *best, hipStream_t s);
 
 // kernel launchers, one pair per (type, dimensionality, reversible):
 // zx_{enc,dec}_<f32|f64|i32|i64>_<1|2|3|4>_<0|1> (zfpx_kernels.hip); the batch
