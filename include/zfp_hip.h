@@ -111,6 +111,25 @@ size_t zfp_hip_batch_wait(zfp_hip_batch *submission, size_t *sizes);
 /* zfp_true when a submission's device work has completed (wait will not block). */
 zfp_bool zfp_hip_batch_ready(const zfp_hip_batch *submission);
 
+/* zfp_compress that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device or pinned memory (code under HIP graph
+ * capture of its fixed-rate work, tight serving loops, harnesses that forbid
+ * allocation).  The same stream bytes, cursor and return value as
+ * zfp_compress, for a field whose data and stream words are both device
+ * memory and that has at most ZFP_HIP_NOALLOC_MAX_BLOCKS blocks; anything
+ * else returns 0 with the stream untouched.  Variable-rate streams take their
+ * per-block lengths, offsets and scan tiles from one static device array per
+ * GPU (about 12 MiB, held until the process ends), shared by every call on
+ * that GPU: calls that overlap on one GPU race on it (this library serializes
+ * its own calls on the host, but not work another process or an unsynchronized
+ * caller leaves on the device).  The offset index is kept only where the
+ * stream's index buffer already has room; otherwise the stream's old index is
+ * marked stale, so a later zfp_decompress walks the stream. */
+#define ZFP_HIP_NOALLOC_MAX_BLOCKS 1048576u
+size_t zfp_hip_compress_noalloc(zfp_stream *stream, const zfp_field *field);
+/* ZFP_HIP_NOALLOC_MAX_BLOCKS, as built into the library. */
+uint64 zfp_hip_noalloc_max_blocks(void);
+
 /* zfp_true when two index tuples of the field may address one element
  * (FieldPlan::strides_may_alias, src/field_plan.rs:92-118): the axes of two
  * or more values, sorted by |stride|, do not each step past the reach of the

@@ -837,16 +837,31 @@ bool stage_field(Workspace &W, const zfp_field *f, const Plan &p, bool copy_in, 
   return true;
 }
 
+// Device scratch of a variable-rate compress supplied by the caller in place
+// of the workspace's (zfp_hip_compress_noalloc): per-block lengths, offsets
+// (blocks + 1) and scan tiles, for at most `blocks` blocks.
+struct Scratch {
+  uint32_t *lens;
+  uint64_t *offs;
+  uint64_t *tiles;
+  uint64_t blocks;
+};
+
 // Encode every block of the field at the cursor of `s`; leaves the cursor
 // where compress + finish would (compress.rs:22-53).  Returns false on a
 // device failure (nothing consistent was written).  `end_bit` receives the
 // bit after the last block; `keep_index` keeps a variable-rate stream's
-// offset index beside it.
+// offset index beside it.  With `sc`, field and stream words must be device
+// memory and nothing is allocated: the scratch is `sc`'s, the index is kept
+// only where the stream's index buffer already holds it (else the old index
+// is marked stale), and the end offset is read back into host stack memory.
 bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st,
-                  ExecParams *xp, bool keep_index = true, uint64_t *end_bit = nullptr)
+                  ExecParams *xp, bool keep_index = true, uint64_t *end_bit = nullptr,
+                  const Scratch *sc = nullptr)
 {
   Workspace &W = ws();
   std::lock_guard<std::mutex> lock(W.mu);
+  if (sc && (!s->device || !is_device_ptr(f->data) || p.nblocks > sc->blocks)) return false;
   const uint64_t start = s->wtell();
   const uint64_t prefix = s->bits ? s->buffer : 0;
   const uint64_t w0 = start / 64;
@@ -866,24 +881,35 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   const bool fixed = fixed_layout(c, p);
   uint64_t end_rel;
   uint64_t *offs = nullptr;
+  uint32_t *lens = nullptr;
   if (fixed) {
     a.fixed = c.maxbits;
     end_rel = a.start + p.nblocks * (uint64_t)c.maxbits;
   } else {
     a.fixed = 0;
-    if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
-    if (!Workspace::grow(W.offs, W.offs_cap, (p.nblocks + 1) * 8)) return false;
-    if (!Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(p.nblocks) * 8)) return false;
-    if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
-    a.lengths = (uint32_t *)W.lens;
+    uint64_t *tiles;
+    if (sc) {
+      lens = sc->lens;
+      offs = sc->offs;
+      tiles = sc->tiles;
+    } else {
+      if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
+      if (!Workspace::grow(W.offs, W.offs_cap, (p.nblocks + 1) * 8)) return false;
+      if (!Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(p.nblocks) * 8)) return false;
+      if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
+      lens = (uint32_t *)W.lens;
+      offs = (uint64_t *)W.offs;
+      tiles = (uint64_t *)W.tiles;
+    }
+    a.lengths = lens;
     a.words = nullptr;
     a.nwords = 0;
     ZCHECK(launch_encode(p.type, p.dims, rev, a, st));
-    offs = (uint64_t *)W.offs;
-    ZCHECK(zx_launch_scan((uint32_t *)W.lens, p.nblocks, a.start, offs, (uint64_t *)W.tiles, st));
-    ZCHECK(hipMemcpyAsync(W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
+    ZCHECK(zx_launch_scan(lens, p.nblocks, a.start, offs, tiles, st));
+    uint64_t end_host = 0;
+    ZCHECK(hipMemcpyAsync(sc ? (void *)&end_host : W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
     ZCHECK(hipStreamSynchronize(st));
-    end_rel = *(uint64_t *)W.one;
+    end_rel = sc ? end_host : *(uint64_t *)W.one;
     a.lengths = nullptr;
     a.offsets = offs;
     a.vbits = end_rel - a.start;
@@ -908,7 +934,10 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   if (!s->device && live) {
     ZCHECK(hipMemcpyAsync(s->words + w0, dwords, live * 8, hipMemcpyDeviceToHost, st));
   }
-  if (!fixed && keep_index) {
+  if (!fixed && keep_index && sc && s->index_cap < p.nblocks) {
+    // no room for the index without allocating: mark the old one stale
+    s->index_n = 0;
+  } else if (!fixed && keep_index) {
     // keep the offset index beside the stream
     if (s->index_cap < p.nblocks) {
       drop_index(s);
@@ -920,7 +949,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
       }
     }
     if (s->index) {
-      ZCHECK(hipMemcpyAsync(s->index, W.lens, p.nblocks * 4, hipMemcpyDeviceToDevice, st));
+      ZCHECK(hipMemcpyAsync(s->index, lens, p.nblocks * 4, hipMemcpyDeviceToDevice, st));
       s->index_n = p.nblocks;
       s->index_start = start;
       s->index_key = cfg_key(c, p);
@@ -2156,6 +2185,70 @@ size_t zfp_compress(zfp_stream *z, const zfp_field *f)
   if (s->overflowed) return 0;
   return s->word_pos * 8;
 }
+
+// ---------------------------------------------------------------- allocation-free compress
+
+// Scratch of zfp_hip_compress_noalloc: one array per GPU, held until the
+// process ends.  Layout (64-bit words): lengths (4 B per block), offsets
+// (blocks + 1), scan tiles.  Nothing in it has to start at zero: the length
+// pass writes every length, the scan every offset and tile.
+constexpr uint64_t kNaBlocks = ZFP_HIP_NOALLOC_MAX_BLOCKS;
+constexpr uint64_t kNaLensWords = kNaBlocks / 2;
+constexpr uint64_t kNaOffsWords = kNaBlocks + 32;
+constexpr uint64_t kNaTileWords = 1024;
+static __device__ __align__(256) uint64_t scratch[kNaLensWords + kNaOffsWords + kNaTileWords];
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current.  Host code must use this address, never the array's
+// name (which on the host is a stand-in, not a device address).
+static uint64_t *noalloc_scratch()
+{
+  static std::mutex mu;
+  static uint64_t *addr[64] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    addr[dev] = (uint64_t *)ptr;
+  }
+  return addr[dev];
+}
+
+// zfp_compress without allocating (no counterpart in the reference; for
+// callers that cannot allocate): device-resident field and stream words, at
+// most ZFP_HIP_NOALLOC_MAX_BLOCKS blocks, scratch from the static array.
+size_t zfp_hip_compress_noalloc(zfp_stream *z, const zfp_field *f)
+{
+  if (!z || !f || !z->stream || !f->data) return 0;
+  Plan p;
+  if (!codec_plan(f, p)) return 0;
+  if (p.nblocks > kNaBlocks || zx_scan_tiles(p.nblocks) > kNaTileWords) return 0;
+  if (((uintptr_t)((const char *)f->data + p.imin * (int64_t)p.esz)) % p.esz) return 0;
+  bitstream *s = z->stream;
+  if (!s->device || !is_device_ptr(f->data)) return 0;
+  const zx::Cfg c = cfg_of(z);
+  if (p.nblocks > 0) {
+    uint64_t *base = noalloc_scratch();
+    if (!base) return 0;
+    const Scratch sc{(uint32_t *)base, base + kNaLensWords, base + kNaLensWords + kNaOffsWords,
+                     kNaBlocks};
+    if (!gpu_compress(s, f, p, c, hip_stream_of(z), nullptr, true, nullptr, &sc)) return 0;
+  } else {
+    s->flush();
+  }
+  if (s->overflowed) return 0;
+  return s->word_pos * 8;
+}
+
+uint64 zfp_hip_noalloc_max_blocks(void) { return kNaBlocks; }
 
 // zfp_decompress (zfp-rs-ffi/src/stream.rs:345-367): bytes read through the
 // aligned end of the last block, or 0 on any failure (truncation included).

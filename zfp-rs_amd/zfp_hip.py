@@ -126,6 +126,8 @@ SIGNATURES = {
     "zfp_stream_kernel_ms": (_D, [_SP, P(_D)]),
     "zfp_hip_device": (ctypes.c_int, []),
     "zfp_hip_field_may_alias": (_B, [_FP]),
+    "zfp_hip_compress_noalloc": (_SZ, [_SP, _FP]),
+    "zfp_hip_noalloc_max_blocks": (_U64, []),
     "zfp_hip_compress_batch": (_SZ, [P(_SP), P(_FP), _SZ, P(_SZ), _U]),
     "zfp_hip_decompress_batch": (_SZ, [P(_SP), P(_FP), _SZ, P(_SZ), _U]),
     "zfp_hip_batch_plan": (_VP, [ctypes.c_int, P(_SP), P(_FP), _SZ, _U]),
