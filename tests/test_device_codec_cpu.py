@@ -170,3 +170,45 @@ def test_block_grid_division_is_exact():
                                       if k * d + r < (1 << 32)], dtype=np.uint64)])
         got = H.divide(n.astype(np.uint32), d)
         assert np.array_equal(got.astype(np.uint64), n // np.uint64(d)), d
+
+
+def random_words(rng, n, density):
+    """n words whose bits are one with probability `density`."""
+    bits = rng.random((n, 64)) < density
+    return np.packbits(bits, axis=1, bitorder="little").view(np.uint64).reshape(-1).copy()
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+@pytest.mark.parametrize("ztype", TYPES)
+def test_random_streams_decode_as_the_oracle(dims, ztype):
+    """Arbitrary bit streams decode as the oracle decodes them: budget cuts
+    inside runs, runs that reach the last coefficient, tests longer than a
+    window and every header, at sparse and dense bit densities."""
+    rng = np.random.default_rng(dims * 10 + ztype)
+    shape = {1: (37,), 2: (9, 13), 3: (5, 6, 7)}[dims]
+    dt = O.DTYPES[ztype]
+    u = np.uint32 if np.dtype(dt).itemsize == 4 else np.uint64
+    cfgs = [O.fixed_rate(r, ztype, dims) for r in (1, 2, 5, 8, 13, 16, 24, 32)]
+    cfgs += [O.fixed_precision(p) for p in (3, 12, 33)]
+    cfgs += [O.fixed_accuracy(1e-3), O.reversible(), O.config(20, 300, 40, -60)]
+    for cfg in cfgs:
+        for rnd in (O.ROUND_NEVER, O.ROUND_LAST):
+            c = O.config(cfg.minbits, cfg.maxbits, cfg.maxprec, cfg.minexp, rnd, False)
+            for density in (0.5, 0.1, 0.02):
+                nblocks = int(np.prod([(v + 3) // 4 for v in shape]))
+                nwords = (nblocks * max(c.maxbits, 64) + 63) // 64 + 8
+                nwords = min(nwords, nblocks * 300 + 64)
+                words = random_words(rng, nwords, density)
+                fixed = c.maxbits if is_fixed(c, ztype, dims) else 0
+                rev = c.minexp < -1074
+                ref, rn, rerr = O.decompress(words, shape, dt, c)
+                lens = None
+                if not fixed:
+                    offs = O.block_offsets(words, shape, dt, c)
+                    lens = np.diff(offs.astype(np.int64)).astype(np.uint32)
+                got, _ = H.decompress(shape, dt, ztype, c, words, 0, fixed, lens, rev, 0)
+                assert np.array_equal(got.view(u), ref.view(u)), (c.minbits, c.maxbits, c.maxprec,
+                                                                  c.minexp, rnd, density)
+                if fixed and fixed <= 4096:
+                    got, _ = H.decompress(shape, dt, ztype, c, words, 0, fixed, None, rev, 3)
+                    assert np.array_equal(got.view(u), ref.view(u))

@@ -194,19 +194,25 @@ template <class I> ZX_HD void fwd_lift(I &x, I &y, I &z, I &w)
   y = wsub(y, (I)(w >> 1));
 }
 
-// inverse lift of one line (transform.rs:98-110)
+// inverse lift of one line (transform.rs:98-110).  Each of its last four
+// pairs, b += a then a -= b - a, leaves a = a - b_old in wrapping arithmetic
+// (a - ((b + a) - a)), which is written out so that each pair is two adds.
 template <class I> ZX_HD void inv_lift(I &x, I &y, I &z, I &w)
 {
   y = wadd(y, (I)(w >> 1));
   w = wsub(w, (I)(y >> 1));
+  I t = y;
   y = wadd(y, w);
-  w = wsub(w, wsub(y, w));
+  w = wsub(w, t);
+  t = z;
   z = wadd(z, x);
-  x = wsub(x, wsub(z, x));
+  x = wsub(x, t);
+  t = y;
   y = wadd(y, z);
-  z = wsub(z, wsub(y, z));
+  z = wsub(z, t);
+  t = w;
   w = wadd(w, x);
-  x = wsub(x, wsub(w, x));
+  x = wsub(x, t);
 }
 
 // reversible lifts (transform.rs:118-142)
