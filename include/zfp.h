@@ -46,6 +46,17 @@
 #define ZFP_HEADER_MODE 0x4u
 #define ZFP_HEADER_FULL 0x7u
 
+/* kinds of stream data (zfp-rs-ffi/include/zfp.h:35-43) */
+#define ZFP_DATA_UNUSED 0x01u
+#define ZFP_DATA_PADDING 0x02u
+#define ZFP_DATA_META 0x04u
+#define ZFP_DATA_MISC 0x08u
+#define ZFP_DATA_PAYLOAD 0x10u
+#define ZFP_DATA_INDEX 0x20u
+#define ZFP_DATA_CACHE 0x40u
+#define ZFP_DATA_HEADER 0x80u
+#define ZFP_DATA_ALL 0xffu
+
 #define ZFP_META_NULL UINT64_MAX
 #define ZFP_MAGIC_BITS 32
 #define ZFP_META_BITS 52
