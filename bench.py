@@ -116,31 +116,15 @@ def batched_leg(n_arrays, reps, seed=4096):
     import torch
 
     import zfp_hip as Z
-    from synth import grf_tiles, grf_window
+    from synth import batch_workload
 
     def note(msg):
         print(f"[batched] {msg}", file=sys.stderr, flush=True)
 
-    rng = np.random.default_rng(seed)
-    tiles = grf_tiles(seed, "cuda")
-    note("field tiles ready")
-    types = [(Z.TYPE_INT32, np.int32, 2.0 ** 16), (Z.TYPE_INT64, np.int64, 2.0 ** 32),
-             (Z.TYPE_FLOAT, np.float32, 1.0), (Z.TYPE_DOUBLE, np.float64, 1.0)]
     arrays, outs, streams, bufs = [], [], [], []
     field_bytes = 0
     classes = set()
-    for i in range(n_arrays):
-        dims = int(rng.integers(1, 4))
-        shape = tuple(int(round(math.exp(rng.uniform(math.log(3), math.log(300))))) for _ in range(dims))
-        zt, npdt, scale = types[int(rng.integers(4))]
-        mode = int(rng.integers(3))
-        if mode == 0:
-            cfg = Z.ZfpConfig.fixed_rate(float(rng.integers(4, 25)), zt, dims)
-        elif mode == 1:
-            cfg = Z.ZfpConfig.fixed_precision(int(rng.integers(8, 49)))
-        else:
-            cfg = Z.ZfpConfig.fixed_accuracy(10.0 ** rng.uniform(-9, -2))
-        x = grf_window(tiles, shape, seed + i, dtype=npdt, scale=scale)
+    for x, cfg, mode in batch_workload(n_arrays, seed, "cuda", note):
         cap = Z.maximum_size(cfg, x)
         buf = torch.zeros((cap + 7) // 8, dtype=torch.int64, device="cuda")
         arrays.append(x)
@@ -148,10 +132,7 @@ def batched_leg(n_arrays, reps, seed=4096):
         bufs.append(buf)
         streams.append(Z.Stream(buf, cfg))
         field_bytes += x.numel() * x.element_size()
-        classes.add((zt, dims, mode == 0))
-        if (i + 1) % 1024 == 0:
-            note(f"{i + 1} arrays generated")
-    del tiles
+        classes.add((str(x.dtype), x.dim(), mode == 0))
     torch.cuda.synchronize()
     note(f"{n_arrays} arrays, {field_bytes / 1e9:.2f} GB")
 
@@ -294,39 +275,24 @@ def serving_leg(n_requests, rate, seed=2000):
 
     import serve
     import zfp_hip as Z
-    from synth import grf_tiles, grf_window
+    from synth import serving_workload
 
     def note(msg):
         print(f"[serving] {msg}", file=sys.stderr, flush=True)
 
-    rng = np.random.default_rng(seed)
-    tiles = grf_tiles(seed, "cuda")
-    types = [(Z.TYPE_INT32, np.int32, 4, 2.0 ** 16), (Z.TYPE_INT64, np.int64, 8, 2.0 ** 32),
-             (Z.TYPE_FLOAT, np.float32, 4, 1.0), (Z.TYPE_DOUBLE, np.float64, 8, 1.0)]
-    trace, t = [], 0.0
-    for i in range(n_requests):
-        dims = int(rng.integers(1, 4))
-        zt, npdt, esz, scale = types[int(rng.integers(4))]
-        nbytes = math.exp(rng.uniform(math.log(16 << 10), math.log(256 << 20)))
-        side = max(3, int(round((nbytes / esz) ** (1.0 / dims))))
-        shape = tuple(max(3, int(side * rng.uniform(0.8, 1.25))) for _ in range(dims))
-        mode = int(rng.integers(3))
-        cfg = (Z.ZfpConfig.fixed_rate(float(rng.integers(4, 25)), zt, dims) if mode == 0 else
-               Z.ZfpConfig.fixed_precision(int(rng.integers(8, 49))) if mode == 1 else
-               Z.ZfpConfig.fixed_accuracy(10.0 ** rng.uniform(-9, -2)))
-        x = grf_window(tiles, shape, seed + i, dtype=npdt, scale=scale)
+    trace = []
+    for kind, x, cfg, t in serving_workload(n_requests, rate, seed):
+        esz = x.element_size()
         cap = Z.maximum_size(cfg, x)
         buf = torch.zeros((cap + 7) // 8, dtype=torch.int64, device="cuda")
         st = Z.Stream(buf, cfg)
-        t += rng.exponential(1.0 / rate)
-        if i % 2:
+        if kind == "decompress":
             st.compress(x)  # the stream a decompress request reads
             st.rewind()
             req = serve.Request("decompress", st, torch.empty_like(x), x.numel() * esz, cap, t)
         else:
             req = serve.Request("compress", st, x, x.numel() * esz, x.numel() * esz + cap, t)
         trace.append(req)
-    del tiles
     torch.cuda.synchronize()
     total = sum(r.nbytes for r in trace)
     note(f"{n_requests} requests, {total / 1e9:.2f} GB, {rate:g} req/s")

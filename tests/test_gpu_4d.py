@@ -174,3 +174,74 @@ def test_batch_codes_4d_arrays_like_single_calls():
         assert torch.equal(b, ref)
     for s in streams:
         s.close()
+
+
+def test_batches_with_3d_and_4d_arrays_in_one_stream():
+    """Batched compress then batched decompress of streams that each carry two
+    arrays in sequence, [3-D, 4-D] in one and [4-D, 3-D] in the other (a
+    stream's later arrays follow its first, in input order), against the
+    same sequence of single calls: words, sizes and decoded values."""
+    a3 = torch.from_numpy(field(O.FLOAT, (9, 10, 11), 7)).cuda()
+    a4 = torch.from_numpy(field(O.FLOAT, (5, 3, 6, 9), 8)).cuda()
+    cfg = Z.ZfpConfig.fixed_precision(18)
+    n = 65536
+    bufs = [torch.zeros(n, dtype=torch.int64, device="cuda") for _ in range(2)]
+    streams = [Z.Stream(b, cfg) for b in bufs]
+    # input order: s0 <- a3, s1 <- a4, s0 <- a4, s1 <- a3
+    order = [(0, a3), (1, a4), (0, a4), (1, a3)]
+    _, sizes = Z._batch(Z.lib().zfp_hip_compress_batch, [streams[i] for i, _ in order],
+                        [a for _, a in order], Z.BATCH_DEVICE_FIELDS)
+    refs = [torch.zeros(n, dtype=torch.int64, device="cuda") for _ in range(2)]
+    rstreams = [Z.Stream(b, cfg) for b in refs]
+    want = [rstreams[i].compress(a) for i, a in order]
+    assert sizes == want
+    for b, r in zip(bufs, refs):
+        assert torch.equal(b, r)
+    # decompress both streams the same way, batched and single
+    for s in streams + rstreams:
+        s.rewind()
+    outs = [torch.zeros_like(a) for _, a in order]
+    _, dsizes = Z._batch(Z.lib().zfp_hip_decompress_batch, [streams[i] for i, _ in order], outs,
+                         Z.BATCH_DEVICE_FIELDS)
+    routs = [torch.zeros_like(a) for _, a in order]
+    dwant = [rstreams[i].decompress(o) for (i, _), o in zip(order, routs)]
+    assert dsizes == dwant
+    for o, r in zip(outs, routs):
+        assert torch.equal(o.view(torch.int32), r.view(torch.int32))
+    for s in streams + rstreams:
+        s.close()
+
+
+def test_refused_plan_keeps_the_stream_index_and_overread():
+    """A plan or submission that the batched launches cannot take (here: one
+    array is 4-D) is refused with every stream as it was: cursor, offset index
+    and overread state (nothing is changed before the batch is accepted)."""
+    cfg = Z.ZfpConfig.fixed_precision(16)
+    a3 = torch.from_numpy(field(O.FLOAT, (12, 12, 12), 9)).cuda()
+    a4 = torch.from_numpy(field(O.FLOAT, (4, 4, 4, 4), 10)).cuda()
+    buf = torch.zeros(8192, dtype=torch.int64, device="cuda")
+    s = Z.Stream(buf, cfg)
+    s.compress(a3)                       # leaves an offset index of 27 blocks
+    L = Z.lib()
+    before = L.zfp_stream_index_export(s.bs, None, 0, None)
+    assert before == 27
+    s.rewind()
+    other = Z.Stream(torch.zeros(8192, dtype=torch.int64, device="cuda"), cfg)
+    f3, f4 = Z.make_field(a3), Z.make_field(a4)
+    SP, FP = ctypes.POINTER(Z.zfp_stream), ctypes.POINTER(Z.zfp_field)
+    zs = (SP * 2)(s.zs, other.zs)
+    fs = (FP * 2)(ctypes.pointer(f3), ctypes.pointer(f4))
+    for encode in (1, 0):
+        assert not L.zfp_hip_batch_plan(encode, zs, fs, 2, Z.BATCH_DEVICE_FIELDS)
+        assert not L.zfp_hip_batch_submit(encode, zs, fs, 2, Z.BATCH_DEVICE_FIELDS, None)
+        assert s.wtell() == 0 and s.rtell() == 0
+        assert L.zfp_stream_index_export(s.bs, None, 0, None) == 27
+    # the kept index still decodes the stream
+    out = torch.zeros_like(a3)
+    s.decompress(out)
+    ref = torch.zeros_like(a3)
+    s2 = Z.Stream(buf, cfg)
+    s2.decompress(ref)                   # a fresh stream: no index, walks
+    assert torch.equal(out.view(torch.int32), ref.view(torch.int32))
+    for x in (s, s2, other):
+        x.close()

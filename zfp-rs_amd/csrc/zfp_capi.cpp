@@ -1381,8 +1381,8 @@ void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, siz
     if (encode) {
       it.start = s->wtell();
     } else {
-      s->reset_overread();
-      it.seek_word0 = s->seek_word;
+      // the seek word reset_overread will leave (batch_commit resets it)
+      it.seek_word0 = s->bits > 0 ? (s->word_pos ? s->word_pos - 1 : 0) : s->word_pos;
       it.start = s->rtell();
     }
     it.w0 = it.start / 64;
@@ -1397,23 +1397,9 @@ void batch_prepare(bool encode, zfp_stream *const *zs, zfp_field *const *fs, siz
     J.prefix = encode && s->bits ? s->buffer : 0;
     J.fixed = fixed ? c.maxbits : 0;
     J.fill = !(fixed && J.start == 0 && (c.maxbits % 64) == 0);
-    if (!fixed) {
-      if (encode) {
-        // keep the offset index beside the stream, as zfp_compress does
-        if (s->index_cap < p.nblocks) {
-          drop_index(s);
-          if (hipMalloc((void **)&s->index, p.nblocks * 4) != hipSuccess) {
-            (void)hipGetLastError();
-            s->index = nullptr;
-          } else {
-            s->index_cap = p.nblocks;
-          }
-        }
-        J.index_out = s->index;
-      } else if (s->index && s->index_n == p.nblocks && s->index_start == it.start &&
-                 (s->index_key == 0 || s->index_key == cfg_key(c, p))) {
-        J.index = s->index;
-      }
+    if (!fixed && !encode && s->index && s->index_n == p.nblocks && s->index_start == it.start &&
+        (s->index_key == 0 || s->index_key == cfg_key(c, p))) {
+      J.index = s->index;
     }
     const int variant = ((int)p.type - (int)zfp_type_int32) * 6 + (p.dims - 1) * 2 + (rev ? 1 : 0);
     it.cls = fixed && p.nblocks >= kSoloBlocks ? kSoloClass + variant : variant * 2 + (fixed ? 0 : 1);
@@ -1661,6 +1647,31 @@ size_t batch_finish(const BatchSet &S, const BatchDev &D, zfp_stream *const *zs,
   return ok;
 }
 
+// What a planned batch changes in its streams before its device work, once
+// the batch is accepted (batch_prepare changes nothing a caller sees): a decode's overread
+// state is reset, as zfp_decompress does; a variable-rate encode gets room for
+// its offset index beside the stream, as zfp_compress keeps it.
+void batch_commit(zfp_stream *const *zs, BatchSet &S)
+{
+  for (BatchItem &it : S.items) {
+    bitstream *s = zs[it.input]->stream;
+    if (!S.encode) {
+      s->reset_overread();
+    } else if (!it.job.fixed) {
+      if (s->index_cap < it.plan.nblocks) {
+        drop_index(s);
+        if (hipMalloc((void **)&s->index, it.plan.nblocks * 4) != hipSuccess) {
+          (void)hipGetLastError();
+          s->index = nullptr;
+        } else {
+          s->index_cap = it.plan.nblocks;
+        }
+      }
+      it.job.index_out = s->index;
+    }
+  }
+}
+
 size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_t count, size_t *sizes,
                  uint flags)
 {
@@ -1668,6 +1679,7 @@ size_t batch_run(bool encode, zfp_stream *const *zs, zfp_field *const *fs, size_
   size_t ok = 0;
   BatchSet S;
   batch_prepare(encode, zs, fs, count, sizes, flags, S);
+  batch_commit(zs, S);
   if (!S.items.empty()) {
     const hipStream_t st = hip_stream_of(zs[S.items[0].input]);
     Workspace &W = ws();
@@ -1826,6 +1838,7 @@ std::unique_ptr<zfp_hip_batch> make_batch(bool encode, zfp_stream *const *stream
     for (size_t i = 0; i < count; i++) restore_cursor(streams[i]->stream, before[i]);
     return nullptr;
   }
+  batch_commit(streams, P->S);
   P->zs.assign(streams, streams + count);
   for (size_t i = 0; i < count; i++) {
     P->fields.push_back(*fields[i]);

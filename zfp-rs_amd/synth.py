@@ -131,3 +131,71 @@ def grf_window(tiles, shape, seed, dtype=np.float32, scale=1.0):
     tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32,
            np.int64: torch.int64}[np.dtype(dtype).type]
     return (w * (sign * scale)).to(tdt).contiguous()
+
+
+# The mixed-array workloads of the batched and served forms: dimensionality
+# uniform over 1-3, scalar type uniform over i32/i64/f32/f64, mode uniform over
+# fixed rate 4-24, fixed precision 8-48 and fixed accuracy 1e-9..1e-2
+# (log-uniform; integer arrays take full precision under fixed accuracy), each
+# array cut from the k^-3 tiles with its own seed.
+_MIX_TYPES = (("int32", np.int32, 2.0 ** 16), ("int64", np.int64, 2.0 ** 32),
+              ("float", np.float32, 1.0), ("double", np.float64, 1.0))
+
+
+def _mix_config(rng, ztype, dims):
+    import zfp_hip as Z
+    mode = int(rng.integers(3))
+    if mode == 0:
+        return Z.ZfpConfig.fixed_rate(float(rng.integers(4, 25)), ztype, dims), mode
+    if mode == 1:
+        return Z.ZfpConfig.fixed_precision(int(rng.integers(8, 49))), mode
+    return Z.ZfpConfig.fixed_accuracy(10.0 ** rng.uniform(-9, -2)), mode
+
+
+def batch_workload(n_arrays, seed=4096, device="cuda", note=None):
+    """The batched workload: `n_arrays` arrays with each axis log-uniform in
+    [3, 300] (many not multiples of 4).  Returns [(array, config, mode)] with
+    mode 0 fixed rate, 1 fixed precision, 2 fixed accuracy."""
+    import math
+
+    import zfp_hip as Z
+    ztypes = {"int32": Z.TYPE_INT32, "int64": Z.TYPE_INT64, "float": Z.TYPE_FLOAT,
+              "double": Z.TYPE_DOUBLE}
+    rng = np.random.default_rng(seed)
+    tiles = grf_tiles(seed, device)
+    out = []
+    for i in range(n_arrays):
+        dims = int(rng.integers(1, 4))
+        shape = tuple(int(round(math.exp(rng.uniform(math.log(3), math.log(300))))) for _ in range(dims))
+        name, npdt, scale = _MIX_TYPES[int(rng.integers(4))]
+        cfg, mode = _mix_config(rng, ztypes[name], dims)
+        out.append((grf_window(tiles, shape, seed + i, dtype=npdt, scale=scale), cfg, mode))
+        if note and (i + 1) % 1024 == 0:
+            note(f"{i + 1} arrays generated")
+    return out
+
+
+def serving_workload(n_requests, rate, seed=2000, device="cuda"):
+    """The served workload: Poisson arrivals at `rate` requests/s, compress and
+    decompress alternating (50/50), array sizes log-uniform from 16 KiB to
+    256 MiB.  Returns [(kind, array, config, arrival_s)]."""
+    import math
+
+    import zfp_hip as Z
+    ztypes = {"int32": Z.TYPE_INT32, "int64": Z.TYPE_INT64, "float": Z.TYPE_FLOAT,
+              "double": Z.TYPE_DOUBLE}
+    rng = np.random.default_rng(seed)
+    tiles = grf_tiles(seed, device)
+    out, t = [], 0.0
+    for i in range(n_requests):
+        dims = int(rng.integers(1, 4))
+        name, npdt, scale = _MIX_TYPES[int(rng.integers(4))]
+        esz = np.dtype(npdt).itemsize
+        nbytes = math.exp(rng.uniform(math.log(16 << 10), math.log(256 << 20)))
+        side = max(3, int(round((nbytes / esz) ** (1.0 / dims))))
+        shape = tuple(max(3, int(side * rng.uniform(0.8, 1.25))) for _ in range(dims))
+        cfg, _ = _mix_config(rng, ztypes[name], dims)
+        x = grf_window(tiles, shape, seed + i, dtype=npdt, scale=scale)
+        t += rng.exponential(1.0 / rate)
+        out.append(("decompress" if i % 2 else "compress", x, cfg, t))
+    return out
