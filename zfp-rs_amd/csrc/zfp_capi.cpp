@@ -896,7 +896,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
       if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
       if (!Workspace::grow(W.offs, W.offs_cap, (p.nblocks + 1) * 8)) return false;
       if (!Workspace::grow(W.tiles, W.tiles_cap, zx_scan_tiles(p.nblocks) * 8)) return false;
-      if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
+      if (!W.one) ZCHECK(hipHostMalloc(&W.one, 32, hipHostMallocDefault));
       lens = (uint32_t *)W.lens;
       offs = (uint64_t *)W.offs;
       tiles = (uint64_t *)W.tiles;
@@ -906,10 +906,13 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
     a.nwords = 0;
     ZCHECK(launch_encode(p.type, p.dims, rev, a, st));
     ZCHECK(zx_launch_scan(lens, p.nblocks, a.start, offs, tiles, st));
-    uint64_t end_host = 0;
-    ZCHECK(hipMemcpyAsync(sc ? (void *)&end_host : W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
+    uint64_t end_host[2] = {0, 0};
+    uint64_t *rb = sc ? end_host : (uint64_t *)W.one;
+    ZCHECK(hipMemcpyAsync(rb, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
+    ZCHECK(hipMemcpyAsync(rb + 1, tiles + 1, 8, hipMemcpyDeviceToHost, st));
     ZCHECK(hipStreamSynchronize(st));
-    end_rel = sc ? end_host : *(uint64_t *)W.one;
+    if (rb[1]) return false;  // the scan's look-back gave up
+    end_rel = rb[0];
     a.lengths = nullptr;
     a.offsets = offs;
     a.vbits = end_rel - a.start;
@@ -1042,7 +1045,7 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
     a.fixed = 0;
     if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
     if (!Workspace::grow(W.offs, W.offs_cap, (p.nblocks + 1) * 8)) return false;
-    if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
+    if (!W.one) ZCHECK(hipHostMalloc(&W.one, 32, hipHostMallocDefault));
     const uint32_t *lens;
     if (s->index && s->index_n == p.nblocks && s->index_start == start &&
         (s->index_key == 0 || s->index_key == cfg_key(c, p))) {
@@ -1055,9 +1058,10 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
     uint64_t *offs = (uint64_t *)W.offs;
     ZCHECK(zx_launch_scan(lens, p.nblocks, a.start, offs, (uint64_t *)W.tiles, st));
     ZCHECK(hipMemcpyAsync(W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
+    ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 2, (uint64_t *)W.tiles + 1, 8, hipMemcpyDeviceToHost, st));
     a.offsets = offs;
   }
-  if (!W.one) ZCHECK(hipHostMalloc(&W.one, 16, hipHostMallocDefault));
+  if (!W.one) ZCHECK(hipHostMalloc(&W.one, 32, hipHostMallocDefault));
   if (!whole) ZCHECK(zx_launch_fill(maxword, 1, 0, st));
   tm.mark(1);
   ZCHECK(launch_decode(p.type, p.dims, rev, a, nullptr, st));
@@ -1086,6 +1090,7 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   tm.mark(3);
   ZCHECK(hipStreamSynchronize(st));
   tm.collect();
+  if (!fixed && ((uint64_t *)W.one)[2]) return false;  // the scan's look-back gave up
   if (!fixed) end_rel = *(uint64_t *)W.one;
   const uint64_t end = 64 * w0 + end_rel;
   end_out = end;

@@ -96,7 +96,8 @@ ZX_DECLARE(i64)
 
 // zfpx_scan.hip
 // offsets[i] = start + sum(lengths[0..i)), offsets[n] = end; `tiles` holds
-// zx_scan_tiles(n) words of scratch.
+// zx_scan_tiles(n) words of scratch, whose word 1 is nonzero after a scan
+// whose look-back gave up (its offsets are then past any stream).
 uint64_t zx_scan_tiles(uint64_t n);
 hipError_t zx_launch_scan(const uint32_t *lengths, uint64_t n, uint64_t start, uint64_t *offsets,
                           uint64_t *tiles, hipStream_t s);
