@@ -394,7 +394,7 @@ def variable_rate_leg(steps=5):
     v3 = torch.randint(-3, 4, (256 ** 3,), generator=g, device="cuda", dtype=torch.int32)
     v3 = torch.cumsum(v3, 0, dtype=torch.int32).reshape(256, 256, 256)
     cases.append(("256^3 i32 random walk reversible", v3, Z.ZfpConfig.reversible(), 0))
-    out = _time_cases(cases, steps, note)
+    out = _time_cases(cases, steps, note, indexless=True)
     del v1, v2, v3, cases
     torch.cuda.empty_cache()
     return out
@@ -428,10 +428,14 @@ def four_d_leg(steps=3):
     return out
 
 
-def _time_cases(cases, steps, note):
+def _time_cases(cases, steps, note, indexless=False):
     """Best compress and decompress time of `steps` (after one untimed step)
     for each (name, device tensor, config, tolerance) case; tolerance 0 checks
-    an exact round trip, a float checks the largest error."""
+    an exact round trip, a float checks the largest error.  Each direction's
+    roofline fraction is its algorithmic bytes (field plus stream) per second
+    over the HBM peak.  `indexless` also times one decode of the same words
+    without their offset index (block lengths read on the host, runs decoded
+    on the GPU as they are found) and checks it equals the indexed decode."""
     import torch
 
     import zfp_hip as Z
@@ -465,7 +469,28 @@ def _time_cases(cases, steps, note):
         s.close()
         r = {"config": name, "uncompressed_bytes": nb, "stream_bytes": n,
              "ratio": round(nb / n, 3), "compress_GBps": round(nb / best_c / 1e9, 2),
-             "decompress_GBps": round(nb / best_d / 1e9, 2), "round_trip_ok": ok}
+             "decompress_GBps": round(nb / best_d / 1e9, 2), "round_trip_ok": ok,
+             "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
+                          "traffic_per_call": nb + n,
+                          "compress_achieved": round((nb + n) / best_c / 1e9, 1),
+                          "compress_frac": round((nb + n) / best_c / 1e9 / HBM_PEAK_GBPS, 4),
+                          "decompress_achieved": round((nb + n) / best_d / 1e9, 1),
+                          "decompress_frac": round((nb + n) / best_d / 1e9 / HBM_PEAK_GBPS, 4)}}
+        if indexless:
+            y2 = torch.empty_like(x)
+            s2 = Z.Stream(buf, cfg)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s2.decompress(y2)
+            torch.cuda.synchronize()
+            t_il = time.perf_counter() - t0
+            s2.close()
+            iv = torch.int32 if x.element_size() == 4 else torch.int64
+            r["indexless_decompress_ms"] = round(t_il * 1e3, 1)
+            r["indexed_decompress_ms"] = round(best_d * 1e3, 3)
+            r["indexless_decompress_GBps"] = round(nb / t_il / 1e9, 2)
+            r["indexless_equals_indexed"] = bool(torch.equal(y2.view(iv), y.view(iv)))
+            del y2
         note(str(r))
         out.append(r)
         del buf, y

@@ -36,6 +36,9 @@ def lib():
         L.hc_alias_scatter.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_int64),
                                        P(ctypes.c_int64), ctypes.c_int64, ctypes.c_int, ctypes.c_uint32]
         L.hc_alias_scatter.restype = ctypes.c_int
+        L.hc_walk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HcCfg), P(ctypes.c_uint64),
+                              ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint64)]
+        L.hc_walk.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -86,3 +89,14 @@ def divide(n, d):
     q = np.empty_like(n)
     lib().hc_divide(n.ctypes.data, int(d), q.ctypes.data, n.size)
     return q
+
+
+def walk(ztype, dims, oc, words, start_bit, count, rev):
+    """The kernels' length reader: (offsets of `count` blocks, end bit)."""
+    offs = np.zeros(count + 1, dtype=np.uint64)
+    c = cfg_of(oc)
+    end = lib().hc_walk(ztype, dims, int(rev), ctypes.byref(c),
+                        words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words), start_bit,
+                        count, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    offs[count] = end
+    return offs

@@ -220,7 +220,36 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
   return 0;
 }
 
+template <class S, bool REV>
+static uint64_t walk_d(int dims, const Cfg &c, const uint64_t *w, uint64_t nw, uint64_t st,
+                       uint64_t count, uint64_t *offs)
+{
+  switch (dims) {
+  case 1: return walk_offsets<S, 1, REV>(w, nw, st, count, c, offs);
+  case 2: return walk_offsets<S, 2, REV>(w, nw, st, count, c, offs);
+  case 3: return walk_offsets<S, 3, REV>(w, nw, st, count, c, offs);
+  default: return walk_offsets<S, 4, REV>(w, nw, st, count, c, offs);
+  }
+}
+
 extern "C" {
+
+// The length reader of index-less decodes: offsets[0..count) and the end bit.
+uint64_t hc_walk(int type, int dims, int rev, const HcCfg *hc, const uint64_t *words,
+                 uint64_t nwords, uint64_t start_bit, uint64_t count, uint64_t *offsets)
+{
+  Cfg c{hc->minbits, hc->maxbits, hc->maxprec, hc->minexp, hc->rounding, hc->tight};
+  switch (type * 2 + (rev ? 1 : 0)) {
+  case 2: return walk_d<int32_t, false>(dims, c, words, nwords, start_bit, count, offsets);
+  case 3: return walk_d<int32_t, true>(dims, c, words, nwords, start_bit, count, offsets);
+  case 4: return walk_d<int64_t, false>(dims, c, words, nwords, start_bit, count, offsets);
+  case 5: return walk_d<int64_t, true>(dims, c, words, nwords, start_bit, count, offsets);
+  case 6: return walk_d<float, false>(dims, c, words, nwords, start_bit, count, offsets);
+  case 7: return walk_d<float, true>(dims, c, words, nwords, start_bit, count, offsets);
+  case 8: return walk_d<double, false>(dims, c, words, nwords, start_bit, count, offsets);
+  default: return walk_d<double, true>(dims, c, words, nwords, start_bit, count, offsets);
+  }
+}
 
 uint64_t hc_compress(int type, int dims, int rev, void *data, const int64_t *n, const int64_t *s,
                      const HcCfg *hc, uint64_t *words, uint64_t nwords, uint64_t start_bit,

@@ -212,3 +212,45 @@ def test_random_streams_decode_as_the_oracle(dims, ztype):
                 if fixed and fixed <= 4096:
                     got, _ = H.decompress(shape, dt, ztype, c, words, 0, fixed, None, rev, 3)
                     assert np.array_equal(got.view(u), ref.view(u))
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3, 4])
+@pytest.mark.parametrize("ztype", TYPES)
+def test_length_reader_matches_the_oracle(dims, ztype):
+    """The length reader of index-less decodes (walk_offsets in zfpx_field.h,
+    which the library runs on the host) finds the block offsets of the
+    oracle's serial decode: on compressed fields of every variable-rate mode,
+    min_bits padding included, from a start bit inside a word, and (1-3-D) on
+    arbitrary bit streams, where budgets cut planes short and runs of zeros
+    reach the last coefficient."""
+    rng = np.random.default_rng(100 + 10 * dims + ztype)
+    shape = {1: (37,), 2: (9, 13), 3: (5, 6, 7), 4: (3, 5, 4, 6)}[dims]
+    dt = O.DTYPES[ztype]
+    nblocks = int(np.prod([(v + 3) // 4 for v in shape]))
+    cfgs = [O.fixed_precision(p) for p in (5, 16, 33, 64)]
+    cfgs += [O.fixed_accuracy(t) for t in (1e-1, 1e-4, 1e-9)]
+    cfgs += [O.reversible(), O.config(1, 11, 64, -1074), O.config(100, 600, 20, -30),
+             O.config(100, 16658, 64, -1075), O.fixed_rate(8, ztype, dims)]
+    kinds = ("smooth", "specials") if np.dtype(dt).kind == "f" else ("smooth", "walk")
+    for cfg in cfgs:
+        rev = cfg.minexp < -1074
+        for kind in kinds:
+            arr = data_for(ztype, shape, kind, seed=dims + 7 * ztype)
+            if dims == 4:
+                words, ref = O.compress_blocks(arr, cfg, 4 * 4 * 16658 // 64 + 64)
+                got = H.walk(ztype, dims, cfg, words, 0, nblocks, rev)
+                assert np.array_equal(got, ref), (cfg.minbits, cfg.maxbits, cfg.maxprec, kind)
+                continue
+            for sb in (0, 13):
+                n, err, words = O.compress(arr, cfg, start_bits=sb)
+                assert err == 0
+                ref = O.block_offsets(words, shape, dt, cfg, start_bits=sb)
+                got = H.walk(ztype, dims, cfg, words, sb, nblocks, rev)
+                assert np.array_equal(got, ref), (cfg.minbits, cfg.maxbits, cfg.maxprec, kind, sb)
+        if dims == 4:
+            continue
+        for density in (0.5, 0.1, 0.02):
+            words = random_words(rng, nblocks * 80 + 64, density)
+            ref = O.block_offsets(words, shape, dt, cfg)
+            got = H.walk(ztype, dims, cfg, words, 0, nblocks, rev)
+            assert np.array_equal(got, ref), (cfg.minbits, cfg.maxbits, cfg.maxprec, density)

@@ -4,9 +4,10 @@ field scaled into [-1e3, 1e3]) at fixed accuracy 1e-6 and fixed precision 32,
 through properties that do not need the oracle to code the whole array: the
 offset index adds up to the stream, 512 sampled blocks are bit-exact against
 the oracle (each block's bits at its indexed offset in the GPU stream equal the
-oracle's code of that block, and its decoded values the oracle's), and the
-whole round trip holds its bound (exact for reversible, within the tolerance
-for fixed accuracy)."""
+oracle's code of that block, and its decoded values the oracle's), the same
+stream decoded without its index (block lengths read on the host) equals the
+indexed decode, and the whole round trip holds its bound (exact for
+reversible, within the tolerance for fixed accuracy)."""
 
 import numpy as np
 import pytest
@@ -100,6 +101,16 @@ def test_full_size_variable_rate_sampled_blocks_match_oracle():
             a = block_view(got, int(b))
             r = ref[..., 4 * k:4 * k + 4]
             assert np.array_equal(np.ascontiguousarray(a).view(u), np.ascontiguousarray(r).view(u)), (name, int(b))
+        # the same words without their offset index: the host reads the block
+        # lengths and the GPU decodes run by run; every value equals the
+        # indexed decode (and so the sampled blocks equal the oracle's)
+        s2 = Z.Stream(buf, cfg)
+        y2 = torch.empty_like(x)
+        assert s2.decompress(y2) == n, name
+        iv = torch.int32 if x.element_size() == 4 else torch.int64
+        assert torch.equal(y2.view(iv), y.view(iv)), name
+        s2.close()
+        del y2
         if tol == 0:
             assert torch.equal(x, y), name
         elif tol is not None:

@@ -61,6 +61,10 @@ struct Workspace {
   void *dense = nullptr; size_t dense_cap = 0;  // decode of an aliasing field
   void *best = nullptr; size_t best_cap = 0;    // its per-element tickets
   void *one = nullptr;  // one pinned host word for reading back a scan total
+  // pinned host memory of the length reader (index-less variable-rate decode):
+  // a copy of a device stream's words, and the block offsets it finds
+  void *hwords = nullptr; size_t hwords_cap = 0;
+  void *hoffs = nullptr; size_t hoffs_cap = 0;
   int device = -1;
 
   static bool grow(void *&p, size_t &cap, size_t need)
@@ -751,7 +755,7 @@ bool contiguous(const Plan &p)
 namespace {
 
 typedef hipError_t (*EncFn)(const zx::EncodeArgs &, hipStream_t);
-typedef hipError_t (*DecFn)(const zx::DecodeArgs &, uint32_t *, hipStream_t);
+typedef hipError_t (*DecFn)(const zx::DecodeArgs &, hipStream_t);
 
 #define ZX_ROW(T) {zx_enc_##T##_1_0, zx_enc_##T##_1_1, zx_enc_##T##_2_0, zx_enc_##T##_2_1, \
                    zx_enc_##T##_3_0, zx_enc_##T##_3_1, zx_enc_##T##_4_0, zx_enc_##T##_4_1}
@@ -762,17 +766,22 @@ const EncFn kEnc[4][8] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
 const DecFn kDec[4][8] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
 #undef ZX_ROW
 
+typedef uint64_t (*WalkFn)(const uint64_t *, uint64_t, uint64_t, uint64_t, const zx::Cfg &, uint64_t *);
+#define ZX_ROW(T) {zx_walk_##T##_1_0, zx_walk_##T##_1_1, zx_walk_##T##_2_0, zx_walk_##T##_2_1, \
+                   zx_walk_##T##_3_0, zx_walk_##T##_3_1, zx_walk_##T##_4_0, zx_walk_##T##_4_1}
+const WalkFn kWalk[4][8] = {ZX_ROW(i32), ZX_ROW(i64), ZX_ROW(f32), ZX_ROW(f64)};
+#undef ZX_ROW
+
 hipError_t launch_encode(zfp_type t, int dims, bool rev, const zx::EncodeArgs &a, hipStream_t s)
 {
   if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 4) return hipErrorInvalidValue;
   return kEnc[t - zfp_type_int32][2 * (dims - 1) + (rev ? 1 : 0)](a, s);
 }
 
-hipError_t launch_decode(zfp_type t, int dims, bool rev, const zx::DecodeArgs &a, uint32_t *walk,
-                         hipStream_t s)
+hipError_t launch_decode(zfp_type t, int dims, bool rev, const zx::DecodeArgs &a, hipStream_t s)
 {
   if (t < zfp_type_int32 || t > zfp_type_double || dims < 1 || dims > 4) return hipErrorInvalidValue;
-  return kDec[t - zfp_type_int32][2 * (dims - 1) + (rev ? 1 : 0)](a, walk, s);
+  return kDec[t - zfp_type_int32][2 * (dims - 1) + (rev ? 1 : 0)](a, s);
 }
 
 uint64_t cfg_key(const zx::Cfg &c, const Plan &p)
@@ -970,6 +979,63 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   return true;
 }
 
+bool grow_host(void *&p, size_t &cap, size_t need)
+{
+  if (need <= cap) return true;
+  const size_t n = std::max(need, cap + cap / 2);
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  if (hipHostMalloc(&p, n, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+    return false;
+  }
+  cap = n;
+  return true;
+}
+
+// Blocks per run of the length reader.
+constexpr uint64_t kWalkRun = 65536;
+
+// Decode of a variable-rate stream without an offset index, pipelined as the
+// reference's index-less decoder is (decompress/pipeline.rs:99-229, a serial
+// plane reader feeding parallel reconstruction): the host reads block
+// lengths in order, a run of kWalkRun blocks at a time, and each run's
+// offsets are copied up and its blocks decoded on the GPU while the host
+// reads the next run.  `a` is set up for the whole field; `need` words of the
+// stream from word w0 on are readable.  end_rel is the end bit of the last
+// block, relative to word w0.
+bool walk_decode(Workspace &W, const bitstream *s, const Plan &p, zx::DecodeArgs a, bool rev,
+                 uint64_t w0, uint64_t need, hipStream_t st, uint64_t &end_rel)
+{
+  const uint64_t *hw = s->words + w0;
+  if (s->device) {
+    if (!grow_host(W.hwords, W.hwords_cap, std::max<uint64_t>(need, 1) * 8)) return false;
+    if (need) ZCHECK(hipMemcpyAsync(W.hwords, s->words + w0, need * 8, hipMemcpyDeviceToHost, st));
+    ZCHECK(hipStreamSynchronize(st));
+    hw = (const uint64_t *)W.hwords;
+  }
+  if (!grow_host(W.hoffs, W.hoffs_cap, (p.nblocks + 1) * 8)) return false;
+  uint64_t *ho = (uint64_t *)W.hoffs;
+  uint64_t *doffs = (uint64_t *)W.offs;
+  const WalkFn walk = kWalk[p.type - zfp_type_int32][2 * (p.dims - 1) + (rev ? 1 : 0)];
+  uint64_t pos = a.start;
+  a.offsets = doffs;
+  for (uint64_t b0 = 0; b0 < p.nblocks; b0 += kWalkRun) {
+    const uint64_t nb = std::min<uint64_t>(kWalkRun, p.nblocks - b0);
+    pos = walk(hw, need, pos, nb, a.cfg, ho + b0);
+    const bool last = b0 + nb == p.nblocks;
+    if (last) ho[p.nblocks] = pos;
+    ZCHECK(hipMemcpyAsync(doffs + b0, ho + b0, (nb + (last ? 1 : 0)) * 8, hipMemcpyHostToDevice, st));
+    a.first = b0;
+    a.count = nb;
+    ZCHECK(launch_decode(p.type, p.dims, rev, a, st));
+  }
+  end_rel = pos;
+  return true;
+}
+
 // Decode every block of the field from the cursor of `s`; leaves the cursor
 // and overread state where decompress + finish would (decompress.rs:27-74).
 bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c, hipStream_t st,
@@ -1037,34 +1103,34 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   }
   a.words = dwords;
   a.nwords = need;
-  uint64_t end_rel;
+  uint64_t end_rel = 0;
+  bool walked = false;  // no offset index: the host reads the block lengths
+  if (!W.one) ZCHECK(hipHostMalloc(&W.one, 32, hipHostMallocDefault));
+  ((uint64_t *)W.one)[2] = 0;
   if (fixed) {
     a.fixed = c.maxbits;
     end_rel = a.start + p.nblocks * (uint64_t)c.maxbits;
   } else {
     a.fixed = 0;
-    if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
     if (!Workspace::grow(W.offs, W.offs_cap, (p.nblocks + 1) * 8)) return false;
-    if (!W.one) ZCHECK(hipHostMalloc(&W.one, 32, hipHostMallocDefault));
-    const uint32_t *lens;
     if (s->index && s->index_n == p.nblocks && s->index_start == start &&
         (s->index_key == 0 || s->index_key == cfg_key(c, p))) {
-      lens = s->index;
+      uint64_t *offs = (uint64_t *)W.offs;
+      ZCHECK(zx_launch_scan(s->index, p.nblocks, a.start, offs, (uint64_t *)W.tiles, st));
+      ZCHECK(hipMemcpyAsync(W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
+      ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 2, (uint64_t *)W.tiles + 1, 8, hipMemcpyDeviceToHost, st));
+      a.offsets = offs;
     } else {
-      // no index: walk the stream on the GPU to find block lengths
-      ZCHECK(launch_decode(p.type, p.dims, rev, a, (uint32_t *)W.lens, st));
-      lens = (const uint32_t *)W.lens;
+      walked = true;
     }
-    uint64_t *offs = (uint64_t *)W.offs;
-    ZCHECK(zx_launch_scan(lens, p.nblocks, a.start, offs, (uint64_t *)W.tiles, st));
-    ZCHECK(hipMemcpyAsync(W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
-    ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 2, (uint64_t *)W.tiles + 1, 8, hipMemcpyDeviceToHost, st));
-    a.offsets = offs;
   }
-  if (!W.one) ZCHECK(hipHostMalloc(&W.one, 32, hipHostMallocDefault));
   if (!whole) ZCHECK(zx_launch_fill(maxword, 1, 0, st));
   tm.mark(1);
-  ZCHECK(launch_decode(p.type, p.dims, rev, a, nullptr, st));
+  if (walked) {
+    if (!walk_decode(W, s, p, a, rev, w0, need, st, end_rel)) return false;
+  } else {
+    ZCHECK(launch_decode(p.type, p.dims, rev, a, st));
+  }
   if (p.may_alias) {
     zx::AliasArgs aa{};
     aa.dense = W.dense;
@@ -1091,7 +1157,7 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
   ZCHECK(hipStreamSynchronize(st));
   tm.collect();
   if (!fixed && ((uint64_t *)W.one)[2]) return false;  // the scan's look-back gave up
-  if (!fixed) end_rel = *(uint64_t *)W.one;
+  if (!fixed && !walked) end_rel = *(uint64_t *)W.one;
   const uint64_t end = 64 * w0 + end_rel;
   end_out = end;
   // park + align: the cursor at the word boundary after the last block.  A
@@ -1539,7 +1605,7 @@ bool batch_enqueue(const BatchSet &S, const BatchDev &D, bool upload, hipStream_
               da.maxword = dres + 2 * q + 1;
               da.vec = J.vec;
               da.slice = D.slice;
-              r = launch_decode(t, dims, rev, da, nullptr, lane());
+              r = launch_decode(t, dims, rev, da, lane());
             }
             if (r != hipSuccess) {
               (void)hipGetLastError();

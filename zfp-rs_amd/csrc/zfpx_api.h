@@ -38,6 +38,8 @@ struct DecodeArgs {
   uint64_t *maxword;        // one past the highest word loaded (written by the last block), or null
   bool vec;
   uint32_t slice;           // blocks per workgroup of the LDS path, or 0 for its default
+  uint64_t first;           // variable placement: decode blocks [first, first + count) only,
+  uint64_t count;           // or every block when count is 0
 };
 
 // One array of a batch (zfp_hip_compress_batch / zfp_hip_decompress_batch).
@@ -78,12 +80,15 @@ hipError_t zx_launch_alias_scatter(const zx::AliasArgs &a, unsigned long long *b
 
 // kernel launchers, one pair per (type, dimensionality, reversible):
 // zx_{enc,dec}_<f32|f64|i32|i64>_<1|2|3|4>_<0|1> (zfpx_kernels.hip); the batch
-// launchers of 4-D return hipErrorNotSupported
+// launchers of 4-D return hipErrorNotSupported.  zx_walk_* is the host's
+// length reader (walk_offsets, zfpx_field.h) over `count` blocks of host words.
 #define ZX_DECLARE1(T, D, R)                                                              \
   hipError_t zx_enc_##T##_##D##_##R(const zx::EncodeArgs &a, hipStream_t s);              \
-  hipError_t zx_dec_##T##_##D##_##R(const zx::DecodeArgs &a, uint32_t *walk, hipStream_t s); \
+  hipError_t zx_dec_##T##_##D##_##R(const zx::DecodeArgs &a, hipStream_t s);              \
   hipError_t zx_benc_##T##_##D##_##R(const zx::BatchArgs &a, bool count, hipStream_t s);  \
-  hipError_t zx_bdec_##T##_##D##_##R(const zx::BatchArgs &a, bool walk, hipStream_t s);
+  hipError_t zx_bdec_##T##_##D##_##R(const zx::BatchArgs &a, bool walk, hipStream_t s);   \
+  uint64_t zx_walk_##T##_##D##_##R(const uint64_t *words, uint64_t nwords, uint64_t pos,   \
+                                   uint64_t count, const zx::Cfg &c, uint64_t *offsets);
 #define ZX_DECLARE(T)                                                                     \
   ZX_DECLARE1(T, 1, 0) ZX_DECLARE1(T, 1, 1) ZX_DECLARE1(T, 2, 0) ZX_DECLARE1(T, 2, 1)     \
   ZX_DECLARE1(T, 3, 0) ZX_DECLARE1(T, 3, 1) ZX_DECLARE1(T, 4, 0) ZX_DECLARE1(T, 4, 1)

@@ -757,6 +757,74 @@ ZX_HD uint64_t expand8_of(uint32_t b)
 #endif
 }
 
+// compress8(b): the one-bits of the byte b, bit i (the r-th one) moved down to
+// i - r; on bytes with no two adjacent one-bits it inverts expand8.
+ZX_HD constexpr uint32_t compress8(uint32_t b)
+{
+  uint32_t v = 0, r = 0;
+  for (uint32_t i = 0; i < 8; i++)
+    if ((b >> i) & 1u) v |= 1u << (i - r++);
+  return v;
+}
+
+struct alignas(8) Compress8Table {
+  uint8_t v[256];
+};
+
+ZX_HD constexpr Compress8Table make_compress8_table()
+{
+  Compress8Table t{};
+  for (uint32_t b = 0; b < 256; b++) t.v[b] = (uint8_t)compress8(b);
+  return t;
+}
+
+static constexpr Compress8Table kCompress8Host = make_compress8_table();
+
+#if ZX_DEVICE_BUILD
+// The table in device memory, and the workgroup's copy in LDS, which kernels
+// that decode fill (compress8_fill) before any thread decodes a block.
+static __constant__ Compress8Table kCompress8Dev = make_compress8_table();
+static __shared__ __align__(8) uint8_t zx_compress8_lds[256];
+#endif
+
+ZX_HD uint32_t compress8_of(uint32_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return zx_compress8_lds[b];
+#else
+  return kCompress8Host.v[b];
+#endif
+}
+
+// The one-bits of v, none adjacent and all below bit `nbits`, the j-th from
+// the bottom moved down by j: byte by byte through the table, byte m placed
+// 8m - (one-bits below byte m) up.  The four low bytes' lookups are issued
+// together; the high four only when a lane of the wave has bits there.
+ZX_HD uint64_t compress_ones(uint64_t v, uint32_t nbits)
+{
+  const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  uint32_t t[8];
+  ZX_UNROLL
+  for (int m = 0; m < 4; m++) t[m] = compress8_of((lo >> (8 * m)) & 0xffu);
+  uint64_t out = t[0];
+  uint32_t cnt = popc32(lo & 0xffu);
+  ZX_UNROLL
+  for (int m = 1; m < 4; m++) {
+    out |= (uint64_t)t[m] << (8 * m - cnt);
+    cnt += popc32((lo >> (8 * m)) & 0xffu);
+  }
+  if (wave_any(nbits > 32)) {
+    ZX_UNROLL
+    for (int m = 4; m < 8; m++) t[m] = compress8_of((hi >> (8 * (m - 4))) & 0xffu);
+    ZX_UNROLL
+    for (int m = 4; m < 8; m++) {
+      out |= (uint64_t)t[m] << (8 * m - cnt);
+      cnt += popc32((hi >> (8 * (m - 4))) & 0xffu);
+    }
+  }
+  return out;
+}
+
 // The one-bits of u (SIZE bits at most), the j-th from the bottom moved up
 // by j, as a 128-bit value (lo, hi).  Bytes are expanded through the table
 // and placed 8m + (one-bits below byte m) up; a byte range that is zero on
@@ -1055,10 +1123,130 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   }
 }
 
-// decode_ints + decode_planes + round_state (bitplane.rs:626-770), written one
-// plane per step as C's decode_ints is; the reference's run-of-empty-planes
-// and whole-plane fast paths leave the same cursor, planes and (m, prec).
-// Returns bits read; `skip` is set when the block is zero and unbiased.
+// The group tests of one plane (bitplane.rs:675-770), parsed a 64-bit window
+// at a time instead of one test at a time, so that a lane's cost does not
+// grow with the plane's one-bits.  The code after the plane's verbatim bits
+// is: a test bit; after a positive test, zeros up to a value one-bit (each
+// zero a coefficient); then the next test.  Past the first test every run of
+// one-bits is therefore value, test, value, test, ..., and the plane ends
+// right after the first run of odd length (a value followed by a negative
+// test).  Run parities come from one carry chain (adding the lowest bit of a
+// run clears the run and sets the bit after it); value bits
+// are the ones at even offsets in their runs, and their coefficients are
+// their positions less the tests below them (compress_ones).  A run that
+// reaches the last coefficient or a budget that ends inside the window is
+// cut with population counts; a plane longer than the window continues in
+// the next, cut where a zero scan is under way.
+//
+// On lanes with `go` the reader sits at the plane's first test, n < SIZE and
+// bits > 0.  x gains the plane's new one-bits; n and bits advance; `one` is
+// whether the last test read was positive (the plane was cut short by the
+// budget or by the last coefficient, round_state's m).
+//
+// Blocks of 4 or 16 values keep a loop of one test per step (plane_steps):
+// their planes hold few one-bits, and a step costs less than a window.
+template <int SIZE, class Src>
+ZX_HD void plane_steps(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &one, bool go)
+{
+  one = false;
+  while (go) {
+    const uint64_t ahead = r.peek();
+    one = (ahead & 1u) != 0;
+    // zeros before the next one-bit, up to the last coefficient (whose
+    // one-bit is implied) or the budget left after the test bit
+    const uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits - 1);  // <= 63
+    const uint32_t zeros = ctz64(ahead >> 1);
+    const bool hit = zeros < limit;
+    const uint32_t adv = hit ? zeros : limit;
+    const uint32_t step = one ? adv + 1 + (hit ? 1u : 0u) : 1u;
+    r.consume(step);
+    bits -= step;
+    x |= (ahead & 1u) << (n + adv);
+    n += one ? adv + 1 : 0u;
+    go = one && bits != 0 && n < (uint32_t)SIZE;
+  }
+}
+
+template <int SIZE, class Src>
+ZX_HD void plane_tests(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &one, bool go)
+{
+  if constexpr (SIZE < 64) {
+    plane_steps<SIZE>(r, n, bits, x, one, go);
+    return;
+  }
+  constexpr uint64_t ODD = 0xaaaaaaaaaaaaaaaaull;
+  uint32_t s = 1;  // the first window starts with the first test bit
+  one = false;
+  while (wave_any(go)) {
+    if (go) {
+      const uint64_t w = r.peek();
+      if (s && !(w & 1u)) {  // negative first test
+        r.consume(1);
+        bits -= 1;
+        go = false;
+      } else {
+        // zero scan from y's bit 0; its top W bits are the stream's
+        const uint64_t y = w >> s;
+        const uint32_t W = 64 - s;
+        // adding the lowest bit of each run that starts at an odd position
+        // clears it; z then marks, in every run, the bits at even offsets
+        // from its start (values), and after every run of odd length, the
+        // bit that follows it
+        const uint64_t z = (y + (y & ~(y << 1) & ODD)) ^ ODD;
+        const uint64_t val = y & z;                                   // value bits
+        const uint64_t term = ~y & (y << 1) & z & lowmask(W);        // negative tests
+        const bool done = term != 0;
+        const uint32_t e = ctz64(term);  // the negative test that ends the plane
+        // y bits consumed; a cut leaves the next window in a zero scan
+        const uint32_t c = done ? e + 1 : W - (uint32_t)((val >> (W - 1)) & 1u);
+        const uint32_t vr = done ? e : c;
+        const uint64_t vals = val & lowmask(vr);
+        const uint64_t xn = compress_ones(vals, vr);
+        // coefficients passed: every bit but the tests (one after each value
+        // but the last before the end, or before the cut)
+        const uint32_t coef = done ? c - popc64(vals) : c - popc64(val & lowmask(c - 1));
+        // a zero scan does not read coefficient SIZE - 1: it is implied
+        const uint32_t q = (uint32_t)SIZE - 1 - n;
+        const bool size_hit = n + coef >= (uint32_t)SIZE;
+        const uint32_t len = size_hit ? q + popc64(xn & lowmask(q)) : c;
+        const uint32_t yb = bits - s;  // budget left for y
+        if (len > yb) {
+          // the budget ends after yb bits: a one is implied unless the last
+          // bit read was a value
+          const uint32_t nt = yb ? yb - popc64(val & lowmask(yb - 1)) : 0u;
+          const bool lastv = yb != 0 && ((val >> (yb - 1)) & 1u);
+          x |= (xn & lowmask(nt)) << n;
+          n += nt;
+          if (!lastv) x |= 1ull << n++;
+          r.consume(s + yb);
+          bits = 0;
+          one = true;
+          go = false;
+        } else if (size_hit) {
+          x |= ((xn & lowmask(q)) << n) | (1ull << (SIZE - 1));
+          n = SIZE;
+          r.consume(s + len);
+          bits -= s + len;
+          one = true;
+          go = false;
+        } else {
+          x |= xn << n;
+          n += coef;
+          r.consume(s + c);
+          bits -= s + c;
+          go = !done;
+        }
+      }
+    }
+    s = 0;
+  }
+}
+
+// decode_ints + decode_planes + round_state (bitplane.rs:626-770), one plane
+// at a time as C's decode_ints is (its group tests by plane_tests); the
+// reference's run-of-empty-planes and whole-plane fast paths leave the same
+// cursor, planes and (m, prec).  Returns bits read; `skip` is set when the
+// block is zero and unbiased.
 template <int P, int SIZE, class Src>
 ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool round_last,
                            typename Ints<P>::U (&u)[SIZE], bool &skip)
@@ -1089,36 +1277,19 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
       k -= live ? 1u : 0u;  // == CH*c + i when live
       m = live ? 0u : m;
       act = live && k >= kmin;
-      pl[i] = 0;
+      uint64_t x = 0;
+      bool tested = false;
       if (act) {
         m = umin(n, bits);
         bits -= m;
-        uint64_t x = r.read(m);
-        // group tests, one one-bit (or the final negative test) per
-        // step, written without branches so the wave's lanes stay in step
-        bool go = bits != 0 && n < (uint32_t)SIZE;
-        const bool tested = go;
-        bool one = false;
-        while (go) {
-          const uint64_t ahead = r.peek();
-          one = (ahead & 1u) != 0;
-          // zeros before the next one-bit, up to the last coefficient (whose
-          // one-bit is implied) or the budget left after the test bit
-          const uint32_t limit = umin((uint32_t)SIZE - 1 - n, bits - 1);  // <= 63
-          const uint32_t zeros = ctz64(ahead >> 1);
-          const bool hit = zeros < limit;
-          const uint32_t adv = hit ? zeros : limit;
-          const uint32_t step = one ? adv + 1 + (hit ? 1u : 0u) : 1u;
-          r.consume(step);
-          bits -= step;
-          x |= (ahead & 1u) << (n + adv);
-          n += one ? adv + 1 : 0u;
-          go = one && bits != 0 && n < (uint32_t)SIZE;
-        }
-        // round_state's m: the count after a plane cut short, or all
-        if (tested) m = one ? n : (uint32_t)SIZE;
-        pl[i] = (PW)x;
+        x = r.read(m);
+        tested = bits != 0 && n < (uint32_t)SIZE;
       }
+      bool one;
+      plane_tests<SIZE>(r, n, bits, x, one, tested);
+      // round_state's m: the count after a plane cut short, or all
+      if (tested) m = one ? n : (uint32_t)SIZE;
+      pl[i] = (PW)x;
     }
     PW any = 0;
     ZX_UNROLL
@@ -1554,6 +1725,91 @@ ZX_HD void decode_reversible(Src &r, typename Ints<ST<S>::P>::U (&b)[Blk<D>::SIZ
     const int e = emax - (P - 2);
     inv_cast_block<S, SIZE>(q, e, b);
   }
+  }
+}
+
+// ---------------------------------------------------------------- length reader
+
+// The bits decode_ints reads, without its values: the plane reader of an
+// index-less variable-rate stream (decode/staged.rs:216-306 reads lengths the
+// same way before reconstructing blocks).  Each plane skips its n verbatim
+// bits, then reads group tests with zero runs counted 64 bits at a time.
+template <int P, int SIZE, class Src>
+ZX_HD uint32_t skip_ints(Src &r, uint32_t maxbits, uint32_t maxprec)
+{
+  const bool constrained = with_maxbits(maxbits, maxprec, SIZE);
+  const uint32_t budget = constrained ? maxbits : 0xffffffffu;
+  const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
+  uint32_t bits = budget, n = 0, k = P;
+  while (bits != 0 && k-- > kmin) {
+    const uint32_t m = umin(n, bits);
+    bits -= m;
+    for (uint32_t left = m; left != 0;) {
+      const uint32_t t = umin(left, 64);
+      r.consume(t);
+      left -= t;
+    }
+    while (bits != 0 && n < (uint32_t)SIZE) {
+      bits -= 1;
+      if (!r.read(1)) break;
+      // zeros up to the next one-bit, the last coefficient (implied one) or
+      // the budget
+      for (uint32_t left = umin((uint32_t)SIZE - 1 - n, bits); left != 0;) {
+        const uint32_t span = umin(left, 64);
+        const uint32_t z = ctz64(r.peek());
+        if (z < span) {
+          r.consume(z + 1);
+          bits -= z + 1;
+          n += z;
+          break;
+        }
+        r.consume(span);
+        bits -= span;
+        n += span;
+        left -= span;
+      }
+      n++;
+    }
+  }
+  return budget - bits;
+}
+
+// Bits of one block's code as decode_lossy / decode_reversible read them
+// (before the caller raises the count to min_bits).
+template <class S, int D, class Src>
+ZX_HD uint32_t skip_lossy(Src &r, const Cfg &c)
+{
+  constexpr int P = ST<S>::P, SIZE = Blk<D>::SIZE;
+  if constexpr (!ST<S>::FLT) {
+    return skip_ints<P, SIZE>(r, c.maxbits, c.maxprec);
+  } else {
+    if (!r.read(1)) return 1;
+    constexpr uint32_t hb = 1 + ST<S>::EBITS;
+    const int emax = (int)r.read(ST<S>::EBITS) - ST<S>::EBIAS;
+    const uint32_t prec = precision_f(emax, c.maxprec, c.minexp, D, c.tight);
+    return hb + skip_ints<P, SIZE>(r, sat_sub(c.maxbits, hb), prec);
+  }
+}
+
+template <int P, int D, class Src>
+ZX_HD uint32_t rev_skip_ints(Src &r, uint32_t maxbits)
+{
+  constexpr uint32_t PB = P == 32 ? 5u : 6u;
+  const uint32_t prec = (uint32_t)r.read(PB) + 1;
+  return PB + skip_ints<P, Blk<D>::SIZE>(r, sat_sub(maxbits, PB), prec);
+}
+
+template <class S, int D, class Src>
+ZX_HD uint32_t skip_reversible(Src &r, const Cfg &c)
+{
+  constexpr int P = ST<S>::P;
+  if constexpr (!ST<S>::FLT) {
+    return rev_skip_ints<P, D>(r, c.maxbits);
+  } else {
+    if (!r.read(1)) return 1;
+    if (r.read(1)) return 2 + rev_skip_ints<P, D>(r, sat_sub(c.maxbits, 2));
+    r.consume(ST<S>::EBITS);
+    return 2 + ST<S>::EBITS + rev_skip_ints<P, D>(r, sat_sub(c.maxbits, 2 + ST<S>::EBITS));
   }
 }
 

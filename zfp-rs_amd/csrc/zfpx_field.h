@@ -200,6 +200,30 @@ ZX_HD void decode_values(Src &r, typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE], 
   else decode_lossy<S, D>(r, v, c);
 }
 
+// Bits block code occupies from the reader's position, min_bits padding
+// included (what decode_values reads, raised as skip_to raises it).
+template <class S, int D, bool REV, class Src>
+ZX_HD uint32_t skip_values(Src &r, const Cfg &c)
+{
+  const uint32_t used = REV ? skip_reversible<S, D>(r, c) : skip_lossy<S, D>(r, c);
+  return used < c.minbits ? c.minbits : used;
+}
+
+// The length reader over a run of blocks: offsets[i] = the start bit of block
+// i of the run, the first at `pos`; returns the bit after the last.
+template <class S, int D, bool REV>
+ZX_HD uint64_t walk_offsets(const uint64_t *words, uint64_t nwords, uint64_t pos, uint64_t count,
+                            const Cfg &c, uint64_t *offsets)
+{
+  for (uint64_t i = 0; i < count; i++) {
+    offsets[i] = pos;
+    Source<GlobalWords> r;
+    r.init(GlobalWords{words, nwords}, pos);
+    pos += skip_values<S, D, REV>(r, c);
+  }
+  return pos;
+}
+
 // Encode block b of the field through `w`; returns its bit count.
 template <class S, int D, bool REV, class Sink>
 ZX_HD uint32_t encode_field_block(const FieldDesc &f, uint64_t b, Sink &w, const Cfg &c)

@@ -13,9 +13,9 @@ hipError_t ZX_FN(enc)(const zx::EncodeArgs &a, hipStream_t s)
   return zx::launch_encode_t<ZX_S, ZX_D, ZX_REV != 0>(a, s);
 }
 
-hipError_t ZX_FN(dec)(const zx::DecodeArgs &a, uint32_t *walk, hipStream_t s)
+hipError_t ZX_FN(dec)(const zx::DecodeArgs &a, hipStream_t s)
 {
-  return zx::launch_decode_t<ZX_S, ZX_D, ZX_REV != 0>(a, walk, s);
+  return zx::launch_decode_t<ZX_S, ZX_D, ZX_REV != 0>(a, s);
 }
 
 hipError_t ZX_FN(benc)(const zx::BatchArgs &a, bool count, hipStream_t s)
@@ -26,4 +26,10 @@ hipError_t ZX_FN(benc)(const zx::BatchArgs &a, bool count, hipStream_t s)
 hipError_t ZX_FN(bdec)(const zx::BatchArgs &a, bool walk, hipStream_t s)
 {
   return zx::launch_batch_decode_t<ZX_S, ZX_D, ZX_REV != 0>(a, walk, s);
+}
+
+uint64_t ZX_FN(walk)(const uint64_t *words, uint64_t nwords, uint64_t pos, uint64_t count,
+                     const zx::Cfg &c, uint64_t *offsets)
+{
+  return zx::walk_offsets<ZX_S, ZX_D, ZX_REV != 0>(words, nwords, pos, count, c, offsets);
 }

@@ -56,6 +56,15 @@ __device__ __forceinline__ void expand8_fill()
   __syncthreads();
 }
 
+// Fills the workgroup's compress8 table (zfpx_block.h); every kernel that
+// decodes calls it first, with all of its threads, before any thread leaves.
+__device__ __forceinline__ void compress8_fill()
+{
+  for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x)
+    ((uint32_t *)zx_compress8_lds)[i] = ((const uint32_t *)kCompress8Dev.v)[i];
+  __syncthreads();
+}
+
 // Rows of four contiguous values are loaded and stored as 16-byte vectors
 // when the host has checked that every row of every whole block is aligned.
 template <class S, int D>
@@ -153,8 +162,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideE
 template <class S, int D, bool REV>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_decode(DecodeArgs a)
 {
-  const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (b >= a.field.nblocks) return;
+  compress8_fill();
+  const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= (a.count ? a.count : a.field.nblocks)) return;
+  const uint64_t b = a.first + i;
   const uint64_t pos = a.offsets ? a.offsets[b] : a.start + b * (uint64_t)a.fixed;
   WordSource r;
   r.init(GlobalWords{a.words, a.nwords}, pos);
@@ -163,26 +174,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideD
   // blocks are contiguous, so the last one loads the highest word
   if (a.maxword && b == a.field.nblocks - 1) *a.maxword = loaded_words(r, pos, a.cfg.minbits);
   store_block<S, D>(a.field, b, a.vec, v);
-}
-
-// Length walk of a stream without an index: one thread decodes the blocks in
-// order, as the reference's serial decoder does, and records each block's
-// length (what it read, raised to min_bits as skip_to does).
-template <class S, int D, bool REV>
-__global__ __launch_bounds__(64) void k_walk(DecodeArgs a, uint32_t *lengths)
-{
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  uint64_t pos = a.start;
-  for (uint64_t b = 0; b < a.field.nblocks; b++) {
-    WordSource r;
-    r.init(GlobalWords{a.words, a.nwords}, pos);
-    typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
-    decode_values<S, D, REV>(r, v, a.cfg);
-    uint64_t used = r.pos() - pos;
-    if (used < a.cfg.minbits) used = a.cfg.minbits;
-    lengths[b] = (uint32_t)used;
-    pos += used;
-  }
 }
 
 // ------------------------------------------------------------------ LDS path
@@ -222,7 +213,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecod
   const uint32_t t = threadIdx.x;
   const FxGroup q = fx_group(blockIdx.x, p.T, a.field.nblocks, a.start, a.fixed);
   for (uint32_t i = t; i < q.nw + 2; i += p.T) fx_dec_stage(lds, a.words, a.nwords, p, q, i);
-  __syncthreads();
+  compress8_fill();
   if (t >= q.nb) return;
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
   const uint64_t loaded = fx_dec_block<S, D, REV, FLAT>(lds, p, q, t, a.fixed, v, a.cfg);
@@ -337,6 +328,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideE
 template <class S, int D, bool REV>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_batch_decode(BatchArgs a)
 {
+  compress8_fill();
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nblocks) return;
   const uint32_t j = job_of(a.firsts, a.njobs, g);
@@ -351,8 +343,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideD
   store_block<S, D>(J.field, b, J.vec, v);
 }
 
-// Index-less variable-rate arrays: one thread per array decodes its blocks in
-// order to find their lengths (as k_walk does for one stream).
+// Index-less variable-rate arrays: one thread per array reads its blocks'
+// lengths in order (skip_values, the length reader the host runs for a single
+// stream).
 template <class S, int D, bool REV>
 __global__ __launch_bounds__(64) void k_batch_walk(BatchArgs a)
 {
@@ -364,11 +357,8 @@ __global__ __launch_bounds__(64) void k_batch_walk(BatchArgs a)
   for (uint64_t b = 0; b < J.field.nblocks; b++) {
     WordSource r;
     r.init(GlobalWords{J.words, J.nwords}, pos);
-    typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
-    decode_values<S, D, REV>(r, v, J.cfg);
-    uint64_t used = r.pos() - pos;
-    if (used < J.cfg.minbits) used = J.cfg.minbits;
-    a.lengths[J.lbase + b] = (uint32_t)used;
+    const uint32_t used = skip_values<S, D, REV>(r, J.cfg);
+    a.lengths[J.lbase + b] = used;
     pos += used;
   }
 }
@@ -413,12 +403,8 @@ static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
 }
 
 template <class S, int D, bool REV>
-static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, hipStream_t s)
+static hipError_t launch_decode_t(const DecodeArgs &a, hipStream_t s)
 {
-  if (walk_lengths) {
-    hipLaunchKernelGGL((k_walk<S, D, REV>), dim3(1), dim3(64), 0, s, a, walk_lengths);
-    return hipGetLastError();
-  }
   if (a.field.nblocks == 0) return hipSuccess;
   if constexpr (D != 4) {
     FxPlan p;
@@ -429,7 +415,7 @@ static hipError_t launch_decode_t(const DecodeArgs &a, uint32_t *walk_lengths, h
       return hipGetLastError();
     }
   }
-  const uint64_t grid = (a.field.nblocks + kThreads - 1) / kThreads;
+  const uint64_t grid = ((a.count ? a.count : a.field.nblocks) + kThreads - 1) / kThreads;
   hipLaunchKernelGGL((k_decode<S, D, REV>), dim3(grid), dim3(kThreads), 0, s, a);
   return hipGetLastError();
 }
