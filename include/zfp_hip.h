@@ -139,6 +139,15 @@ uint64 zfp_hip_noalloc_max_blocks(void);
  * zfp_false for a null or invalid field. */
 zfp_bool zfp_hip_field_may_alias(const zfp_field *field);
 
+/* Test hook, no counterpart in the C ABI (zfp-rs injects the same kind of
+ * failure in its tests, src/decompress/pipeline.rs:76-94 FAIL_BUFFER_AFTER):
+ * from this call on, the n-th reservation of workspace, staging or pinned
+ * memory (0 = the next) fails once as an out-of-memory would; a call that
+ * meets it returns 0 (or NULL) before it touches the caller's stream or
+ * field.  A negative n disarms.  Returns the count that was left: -1 when
+ * the armed failure has been used or none was armed. */
+long zfp_hip_fail_after(long n);
+
 /* Device the library runs on (the current HIP device when first used). */
 int zfp_hip_device(void);
 

@@ -51,6 +51,19 @@ bool is_device_ptr(const void *p)
 }
 
 // Grow-only device buffers shared by the codec calls of this process.
+// Injected reservation failures for tests (zfp_hip_fail_after): counts the
+// reservations of workspace, staging and pinned memory from the moment it is
+// armed, and makes the n-th fail as an out-of-memory would, once.
+std::atomic<long> g_fail_after{-1};
+
+bool injected_failure()
+{
+  long n = g_fail_after.load();
+  while (n >= 0 && !g_fail_after.compare_exchange_weak(n, n - 1)) {
+  }
+  return n == 0;
+}
+
 struct Workspace {
   std::mutex mu;
   void *field = nullptr; size_t field_cap = 0;
@@ -69,6 +82,7 @@ struct Workspace {
 
   static bool grow(void *&p, size_t &cap, size_t need)
   {
+    if (injected_failure()) return false;
     if (need <= cap) return true;
     const size_t n = std::max(need, cap + cap / 2);
     if (p) (void)hipFree(p);
@@ -981,6 +995,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
 
 bool grow_host(void *&p, size_t &cap, size_t need)
 {
+  if (injected_failure()) return false;
   if (need <= cap) return true;
   const size_t n = std::max(need, cap + cap / 2);
   if (p) (void)hipHostFree(p);
@@ -1361,6 +1376,7 @@ BatchBuffers &batch_buffers()
 
 bool grow_pinned(void *&p, size_t &cap, size_t need)
 {
+  if (injected_failure()) return false;
   if (need <= cap) return true;
   const size_t n = std::max(need, cap + cap / 2);
   if (p) (void)hipHostFree(p);
@@ -2968,6 +2984,11 @@ size_t zfp_hip_batch_wait(zfp_hip_batch *P, size_t *sizes)
     return 0;
   }
   return batch_finish(P->S, P->D, P->zs.data(), sizes);
+}
+
+long zfp_hip_fail_after(long n)
+{
+  return g_fail_after.exchange(n < 0 ? -1 : n);
 }
 
 int zfp_hip_device(void)

@@ -125,6 +125,7 @@ SIGNATURES = {
     "zfp_stream_set_timing": (_B, [_SP, _B]),
     "zfp_stream_kernel_ms": (_D, [_SP, P(_D)]),
     "zfp_hip_device": (ctypes.c_int, []),
+    "zfp_hip_fail_after": (ctypes.c_long, [ctypes.c_long]),
     "zfp_hip_field_may_alias": (_B, [_FP]),
     "zfp_hip_compress_noalloc": (_SZ, [_SP, _FP]),
     "zfp_hip_noalloc_max_blocks": (_U64, []),
