@@ -36,6 +36,11 @@ def lib():
         L.hc_alias_scatter.argtypes = [ctypes.c_void_p, ctypes.c_void_p, P(ctypes.c_int64),
                                        P(ctypes.c_int64), ctypes.c_int64, ctypes.c_int, ctypes.c_uint32]
         L.hc_alias_scatter.restype = ctypes.c_int
+        L.hc_compress_1p.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                     P(ctypes.c_int64), P(ctypes.c_int64), P(HcCfg),
+                                     P(ctypes.c_uint64), ctypes.c_uint64, ctypes.c_uint64,
+                                     P(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32]
+        L.hc_compress_1p.restype = ctypes.c_uint64
         L.hc_walk.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HcCfg), P(ctypes.c_uint64),
                               ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint64)]
         L.hc_walk.restype = ctypes.c_uint64
@@ -100,3 +105,17 @@ def walk(ztype, dims, oc, words, start_bit, count, rev):
                         count, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
     offs[count] = end
     return offs
+
+
+def compress_1p(arr, ztype, oc, words, start_bit, rev, T=0, cap=0):
+    """The one-pass variable-rate compress (T, cap: 0 for the plan's)."""
+    n, s = _dims(arr)
+    nb = 1
+    for v in list(reversed(arr.shape)):
+        nb *= (v + 3) // 4
+    lens = np.zeros(nb, dtype=np.uint32)
+    c = cfg_of(oc)
+    end = lib().hc_compress_1p(ztype, arr.ndim, int(rev), arr.ctypes.data, n, s, ctypes.byref(c),
+                               words.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(words),
+                               start_bit, lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), T, cap)
+    return end, lens

@@ -169,6 +169,77 @@ static uint64_t compress_vx(const FieldDesc &f, const Cfg &c, uint64_t *words, u
   return offs[f.nblocks];
 }
 
+// The one-pass compress (zfpx_fx.h op_*): each workgroup's phases one thread
+// after another, the workgroups in ticket order (so each group's look-back
+// prefix is the running end), then the merge of the shared edge words.
+// Stream words are not cleared first: the path writes every word of the
+// stream's range itself, and the caller's bits below `start` come from the
+// word that holds them, as the cursor's buffer holds them.
+template <class S, int D, bool REV>
+static uint64_t compress_1p(const FieldDesc &f, const Cfg &c, uint64_t *words, uint64_t nwords,
+                            uint64_t start, uint32_t *lens, uint32_t T, uint32_t cap)
+{
+  typedef typename Ints<ST<S>::P>::U U;
+  const OnePlan p = op_plan(block_bound(c, ST<S>::P, Blk<D>::SIZE), 0, T, cap);
+  const uint64_t prefix = (start & 63) ? words[start >> 6] & lowmask((uint32_t)(start & 63)) : 0;
+  const uint64_t groups = (f.nblocks + p.T - 1) / p.T;
+  std::vector<uint64_t> lds(p.lds_words), edges(2 * groups), ends(groups);
+  std::vector<PreparedBlock<S, D>> pb(p.T);
+  std::vector<uint32_t> len(p.T);
+  uint64_t pos = start;
+  for (uint64_t g = 0; g < groups; g++) {
+    uint32_t total = 0;
+    for (uint32_t t = 0; t < p.T; t++) {
+      const uint64_t b = g * p.T + t;
+      len[t] = 0;
+      if (b >= f.nblocks) continue;
+      U v[Blk<D>::SIZE];
+      gather<S, D>(f, block_pos<D>(f, b), v);
+      prepare_block<S, D, REV>(v, c, pb[t]);
+      CountSink cs{0};
+      len[t] = code_block<S, D>(cs, pb[t], c);
+      lens[b] = len[t];
+      total += len[t];
+    }
+    const OneRange r = op_range(pos, total);
+    uint64_t edge[2] = {0, 0};
+    uint32_t off = 0;
+    if (r.nw <= p.cap) {
+      for (auto &x : lds) x = 0x5555aaaa5555aaaaull;  // LDS starts undefined
+      for (uint32_t i = 0; i < r.nw; i++) lds[fx_slot(i, p.skew)] = 0;
+      for (uint32_t t = 0; t < p.T && g * p.T + t < f.nblocks; t++) {
+        op_stage<S, D>(lds.data(), p, (uint32_t)(r.b0 & 63) + off, pb[t], c);
+        off += len[t];
+      }
+      for (uint32_t i = 0; i < r.nw; i++) op_store(words, nwords, lds.data(), p, r, i, edge);
+    } else {
+      for (uint32_t i = 0; i < r.nw; i++) op_clear(words, nwords, r, i);
+      for (uint32_t t = 0; t < p.T && g * p.T + t < f.nblocks; t++) {
+        op_direct<S, D>(words, nwords, r, r.b0 + off, pb[t], c, edge);
+        off += len[t];
+      }
+    }
+    edges[2 * g] = edge[0];
+    edges[2 * g + 1] = edge[1];
+    pos = r.b1;
+    ends[g] = pos;
+  }
+  for (uint64_t g = 0; g < groups; g++)
+    op_merge(words, nwords, ends.data(), edges.data(), g, groups, start, prefix);
+  return pos;
+}
+
+template <class S, bool REV>
+static uint64_t comp1p_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, uint64_t nw,
+                         uint64_t st, uint32_t *l, uint32_t T, uint32_t cap)
+{
+  switch (dims) {
+  case 1: return compress_1p<S, 1, REV>(f, c, w, nw, st, l, T, cap);
+  case 2: return compress_1p<S, 2, REV>(f, c, w, nw, st, l, T, cap);
+  default: return compress_1p<S, 3, REV>(f, c, w, nw, st, l, T, cap);
+  }
+}
+
 template <class S, bool REV>
 static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, uint64_t nw,
                        uint64_t st, uint32_t fx, uint32_t *l, uint32_t T, uint32_t cap)
@@ -233,6 +304,26 @@ static uint64_t walk_d(int dims, const Cfg &c, const uint64_t *w, uint64_t nw, u
 }
 
 extern "C" {
+
+// The one-pass variable-rate compress of a 1-3-D field; T and cap override
+// the plan's workgroup size and LDS capacity when nonzero.
+uint64_t hc_compress_1p(int type, int dims, int rev, void *data, const int64_t *n, const int64_t *s,
+                        const HcCfg *hc, uint64_t *words, uint64_t nwords, uint64_t start_bit,
+                        uint32_t *lengths, uint32_t T, uint32_t cap)
+{
+  FieldDesc f = make_field(dims, data, n, s);
+  Cfg c{hc->minbits, hc->maxbits, hc->maxprec, hc->minexp, hc->rounding, hc->tight};
+  switch (type * 2 + (rev ? 1 : 0)) {
+  case 2: return comp1p_d<int32_t, false>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  case 3: return comp1p_d<int32_t, true>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  case 4: return comp1p_d<int64_t, false>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  case 5: return comp1p_d<int64_t, true>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  case 6: return comp1p_d<float, false>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  case 7: return comp1p_d<float, true>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  case 8: return comp1p_d<double, false>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  default: return comp1p_d<double, true>(dims, f, c, words, nwords, start_bit, lengths, T, cap);
+  }
+}
 
 // The length reader of index-less decodes: offsets[0..count) and the end bit.
 uint64_t hc_walk(int type, int dims, int rev, const HcCfg *hc, const uint64_t *words,

@@ -254,3 +254,43 @@ def test_length_reader_matches_the_oracle(dims, ztype):
             ref = O.block_offsets(words, shape, dt, cfg)
             got = H.walk(ztype, dims, cfg, words, 0, nblocks, rev)
             assert np.array_equal(got, ref), (cfg.minbits, cfg.maxbits, cfg.maxprec, density)
+
+
+@pytest.mark.parametrize("dims", [1, 2, 3])
+@pytest.mark.parametrize("ztype", TYPES)
+def test_one_pass_compress_matches_oracle(dims, ztype):
+    """The one-pass variable-rate compress (prepare and count each block,
+    scan, look back, code from the same registers, merge the words groups
+    share) gives the oracle's stream and block lengths for every
+    variable-rate mode and rounding, from a start bit inside a word with the
+    caller's bits below it, into words it does not clear first: with the
+    plan's workgroups, with groups of 64 and 3 blocks, and with an LDS room of
+    3 words, which sends most groups straight to global memory."""
+    rng = np.random.default_rng(7 * dims + ztype)
+    cfgs = [c for name, c in configs(ztype, dims) if not is_fixed(c, ztype, dims)]
+    for rnd, tight in ROUNDINGS[1:]:
+        b = O.fixed_precision(20)
+        cfgs.append(O.config(b.minbits, b.maxbits, b.maxprec, b.minexp, rnd, tight))
+    kinds = ("smooth", "specials") if ztype in (O.FLOAT, O.DOUBLE) else ("smooth", "walk")
+    for shape in SHAPES[dims]:
+        for kind in kinds:
+            arr = data_for(ztype, shape, kind, seed=int(rng.integers(1 << 30)))
+            for cfg in cfgs:
+                rev = cfg.minexp < -1074
+                for sb in (0, 77):
+                    n, err, ow = O.compress(arr, cfg, start_bits=sb)
+                    assert err == 0
+                    nwords = n // 8
+                    offs = O.block_offsets(ow, shape, arr.dtype, cfg, start_bits=sb)
+                    for T, cap in ((0, 0), (64, 0), (3, 0), (3, 3), (64, 3)):
+                        words = np.full(nwords + 4, 0xa5a5a5a5a5a5a5a5, dtype=np.uint64)
+                        words[: sb // 64] = PREFIX_WORD
+                        if sb % 64:
+                            words[sb // 64] = PREFIX_PART & ((1 << (sb % 64)) - 1) | (0xf0f0 << (sb % 64))
+                        end, lens = H.compress_1p(arr, ztype, cfg, words, sb, rev, T, cap)
+                        ctx = (shape, kind, cfg.minbits, cfg.maxbits, cfg.maxprec, cfg.minexp,
+                               cfg.rounding, sb, T, cap)
+                        assert end == offs[-1], ctx
+                        assert np.array_equal(lens, np.diff(offs.astype(np.int64)).astype(np.uint32)), ctx
+                        assert np.array_equal(words[:nwords], ow[:nwords]), ctx
+                        assert (words[nwords:] == 0xa5a5a5a5a5a5a5a5).all(), ctx

@@ -130,6 +130,9 @@ struct bitstream {
   // the batch call that last took this stream (duplicates within one call
   // go to the single-array calls)
   uint64_t batch_epoch;
+  // mean block length in bits of the last one-pass compress through this
+  // stream (0: none yet), which sizes the next one's LDS staging
+  uint32_t mean_bits;
 
   uint64_t get(size_t i) const
   {
@@ -862,13 +865,44 @@ bool stage_field(Workspace &W, const zfp_field *f, const Plan &p, bool copy_in, 
 
 // Device scratch of a variable-rate compress supplied by the caller in place
 // of the workspace's (zfp_hip_compress_noalloc): per-block lengths, offsets
-// (blocks + 1) and scan tiles, for at most `blocks` blocks.
+// (blocks + 1; the one-pass state and edges use the same words) and scan
+// tiles, for at most `blocks` blocks.
 struct Scratch {
   uint32_t *lens;
   uint64_t *offs;
   uint64_t *tiles;
   uint64_t blocks;
+  uint64_t offs_words;
 };
+
+// Keeps the block lengths `lens` (device) as the stream's offset index; with
+// `no_alloc`, only where the stream's index buffer already has room (else the
+// old index is marked stale).  A failed allocation leaves no index.
+void keep_offset_index(bitstream *s, const uint32_t *lens, const Plan &p, const zx::Cfg &c, uint64_t start,
+                       bool no_alloc, hipStream_t st)
+{
+  if (no_alloc && s->index_cap < p.nblocks) {
+    s->index_n = 0;
+    return;
+  }
+  if (s->index_cap < p.nblocks) {
+    drop_index(s);
+    if (hipMalloc((void **)&s->index, p.nblocks * 4) != hipSuccess) {
+      (void)hipGetLastError();
+      s->index = nullptr;
+      return;
+    }
+    s->index_cap = p.nblocks;
+  }
+  if (hipMemcpyAsync(s->index, lens, p.nblocks * 4, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+    (void)hipGetLastError();
+    s->index_n = 0;
+    return;
+  }
+  s->index_n = p.nblocks;
+  s->index_start = start;
+  s->index_key = cfg_key(c, p);
+}
 
 // Encode every block of the field at the cursor of `s`; leaves the cursor
 // where compress + finish would (compress.rs:22-53).  Returns false on a
@@ -905,6 +939,77 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   uint64_t end_rel;
   uint64_t *offs = nullptr;
   uint32_t *lens = nullptr;
+  // variable-rate 1-3-D fields take the one-pass kernel
+  const int size = 1 << (2 * p.dims);
+  const int tbits = p.type == zfp_type_int32 || p.type == zfp_type_float ? 32 : 64;
+  if (!fixed && p.dims <= 3) {
+    // workgroup size and staging from the stream's last mean block length
+    const zx::OnePlan op = zx::op_plan(zx::block_bound(c, tbits, size), s->mean_bits, 0, 0);
+    const uint64_t groups = (p.nblocks + op.T - 1) / op.T;
+    uint64_t *state, *edges;
+    if (sc) {
+      if (2 + 3 * groups > sc->offs_words) return false;
+      lens = sc->lens;
+      state = sc->offs;
+      edges = sc->offs + 2 + groups;
+    } else {
+      if (!Workspace::grow(W.lens, W.lens_cap, p.nblocks * 4)) return false;
+      if (!Workspace::grow(W.tiles, W.tiles_cap, (2 + groups) * 8)) return false;
+      if (!Workspace::grow(W.offs, W.offs_cap, 2 * groups * 8)) return false;
+      if (!W.one) ZCHECK(hipHostMalloc(&W.one, 32, hipHostMallocDefault));
+      lens = (uint32_t *)W.lens;
+      state = (uint64_t *)W.tiles;
+      edges = (uint64_t *)W.offs;
+    }
+    const uint64_t cap_words = s->nwords > w0 ? s->nwords - w0 : 0;
+    uint64_t *dwords;
+    uint64_t nwords = cap_words;
+    if (s->device) {
+      dwords = s->words + w0;
+    } else {
+      // staging for the words the stream can hold, at most every block's largest code
+      const uint64_t bound = a.start + p.nblocks * (uint64_t)zx::block_bound(c, tbits, size);
+      nwords = std::min<uint64_t>(cap_words, (bound + 63) / 64);
+      if (!Workspace::grow(W.words, W.words_cap, std::max<uint64_t>(nwords, 1) * 8)) return false;
+      dwords = (uint64_t *)W.words;
+    }
+    ZCHECK(zx_launch_fill(state, 2 + groups, 0, st));
+    a.fixed = 0;
+    a.words = dwords;
+    a.nwords = nwords;
+    a.lengths = lens;
+    a.op_state = state;
+    a.op_edges = edges;
+    a.op_T = op.T;
+    a.vcap = op.cap;
+    tm.mark(1);
+    ZCHECK(launch_encode(p.type, p.dims, rev, a, st));
+    ZCHECK(zx_launch_op_merge(dwords, nwords, state, edges, groups, a.start, prefix, st));
+    tm.mark(2);
+    uint64_t end_host[2] = {0, 0};
+    uint64_t *rb = sc ? end_host : (uint64_t *)W.one;
+    ZCHECK(hipMemcpyAsync(rb, state + 2 + groups - 1, 8, hipMemcpyDeviceToHost, st));
+    ZCHECK(hipMemcpyAsync(rb + 1, state + 1, 8, hipMemcpyDeviceToHost, st));
+    if (keep_index) keep_offset_index(s, lens, p, c, start, sc != nullptr, st);
+    tm.mark(3);
+    ZCHECK(hipStreamSynchronize(st));
+    tm.collect();
+    if (rb[1]) return false;  // a look-back gave up
+    end_rel = rb[0] & ((1ull << 62) - 1);
+    s->mean_bits = (uint32_t)std::min<uint64_t>((end_rel - a.start + p.nblocks - 1) / p.nblocks, 1u << 20);
+    const uint64_t w1 = (64 * w0 + end_rel + 63) / 64;  // words through the flush
+    const uint64_t live = std::min<uint64_t>(w1 - w0, nwords);
+    if (!s->device && live) {
+      ZCHECK(hipMemcpyAsync(s->words + w0, dwords, live * 8, hipMemcpyDeviceToHost, st));
+      ZCHECK(hipStreamSynchronize(st));
+    }
+    if (end_bit) *end_bit = 64 * w0 + end_rel;
+    if (w1 > s->nwords) s->overflowed = true;
+    s->word_pos = (size_t)w1;
+    s->bits = 0;
+    s->buffer = 0;
+    return true;
+  }
   if (fixed) {
     a.fixed = c.maxbits;
     end_rel = a.start + p.nblocks * (uint64_t)c.maxbits;
@@ -960,27 +1065,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   if (!s->device && live) {
     ZCHECK(hipMemcpyAsync(s->words + w0, dwords, live * 8, hipMemcpyDeviceToHost, st));
   }
-  if (!fixed && keep_index && sc && s->index_cap < p.nblocks) {
-    // no room for the index without allocating: mark the old one stale
-    s->index_n = 0;
-  } else if (!fixed && keep_index) {
-    // keep the offset index beside the stream
-    if (s->index_cap < p.nblocks) {
-      drop_index(s);
-      if (hipMalloc((void **)&s->index, p.nblocks * 4) != hipSuccess) {
-        (void)hipGetLastError();
-        s->index = nullptr;
-      } else {
-        s->index_cap = p.nblocks;
-      }
-    }
-    if (s->index) {
-      ZCHECK(hipMemcpyAsync(s->index, lens, p.nblocks * 4, hipMemcpyDeviceToDevice, st));
-      s->index_n = p.nblocks;
-      s->index_start = start;
-      s->index_key = cfg_key(c, p);
-    }
-  }
+  if (!fixed && keep_index) keep_offset_index(s, lens, p, c, start, sc != nullptr, st);
   tm.mark(3);
   ZCHECK(hipStreamSynchronize(st));
   tm.collect();
@@ -2339,7 +2424,7 @@ size_t zfp_hip_compress_noalloc(zfp_stream *z, const zfp_field *f)
     uint64_t *base = noalloc_scratch();
     if (!base) return 0;
     const Scratch sc{(uint32_t *)base, base + kNaLensWords, base + kNaLensWords + kNaOffsWords,
-                     kNaBlocks};
+                     kNaBlocks, kNaOffsWords};
     if (!gpu_compress(s, f, p, c, hip_stream_of(z), nullptr, true, nullptr, &sc)) return 0;
   } else {
     s->flush();

@@ -24,7 +24,13 @@ struct EncodeArgs {
   uint64_t vbits;           // variable placement: bits of all blocks (offsets[nblocks] -
                             // offsets[0]), sizing the LDS path; 0 keeps to global memory
   uint32_t vcap;            // variable placement: words a workgroup may stage, or 0 for
-                            // the default (from vbits)
+                            // the default (from vbits; one pass: the largest group)
+  // one-pass variable-rate compress (op_* in zfpx_fx.h), when op_state is set;
+  // `lengths` then receives each block's bit count (the offset index), or is null
+  uint64_t *op_state;       // [0] ticket counter, [1] failure flag, [2 + g] group g's
+                            // look-back status; cleared before the launch
+  uint64_t *op_edges;       // two words per group: its shared first and last words
+  uint32_t op_T;            // blocks per workgroup, or 0 for the default
 };
 
 struct DecodeArgs {
@@ -108,6 +114,13 @@ hipError_t zx_launch_scan(const uint32_t *lengths, uint64_t n, uint64_t start, u
                           uint64_t *tiles, hipStream_t s);
 // words[0] = first, words[1..n) = 0
 hipError_t zx_launch_fill(uint64_t *words, uint64_t n, uint64_t first, hipStream_t s);
+// The second launch of the one-pass compress: every stream word shared by two
+// groups (or by the first group and the caller's `prefix` bits below `start`)
+// from the groups' edges; `state` is the one-pass state after its launch.
+hipError_t zx_launch_op_merge(uint64_t *words, uint64_t nwords, const uint64_t *state,
+                              const uint64_t *edges, uint64_t groups, uint64_t start, uint64_t prefix,
+                              hipStream_t s);
+
 
 // Batch helpers over all `n` arrays of a batch.  Per-array work is cut into
 // chunks of zx_batch_chunks(items) workgroups; first[j] is array j's first

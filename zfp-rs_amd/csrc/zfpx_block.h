@@ -594,6 +594,64 @@ struct LdsSink {
   }
 };
 
+// WordSink for the blocks of a group written straight to global memory (the
+// one-pass compress, for a group too large to stage): the group's first and
+// last words, which it shares with the neighbouring groups, go to two edge
+// words (LDS on the device) by atomic OR instead of the stream; the caller
+// has zeroed the group's other words that blocks share.
+struct EdgeSink {
+  static constexpr bool kSpillPastBudget = false;
+  WordSink ws;
+  uint64_t head, tail;  // stream word indices of the group's shared edge words, or ~0
+  uint64_t *edge;       // edge[0]: head word bits, edge[1]: tail word bits
+
+  ZX_HD void init(uint64_t *w, uint64_t capacity, uint64_t bitpos, uint64_t h, uint64_t t, uint64_t *e)
+  {
+    ws.init(w, capacity, bitpos);
+    head = h;
+    tail = t;
+    edge = e;
+  }
+  ZX_HD void out(uint64_t i, uint64_t v, bool shared)
+  {
+    if (i == head || i == tail) {
+      uint64_t *p = edge + (i == head ? 0 : 1);
+#if defined(__HIP_DEVICE_COMPILE__)
+      atomicOr((unsigned long long *)p, (unsigned long long)v);
+#else
+      *p |= v;
+#endif
+    } else if (i < ws.cap) {
+      if (shared) WordSink::merge(ws.words + i, v);
+      else ws.words[i] = v;
+    }
+  }
+  ZX_HD void put128(uint64_t lo, uint64_t hi, uint32_t n)
+  {
+    uint64_t w0, w1;
+    const uint32_t full = append128(ws.buf, ws.bits, lo, hi, n, w0, w1);
+    if (full >= 1) {
+      out(ws.idx, w0, ws.shared_first);
+      ws.shared_first = false;
+      ws.idx++;
+    }
+    if (full >= 2) {
+      out(ws.idx, w1, false);
+      ws.idx++;
+    }
+  }
+  ZX_HD void put(uint64_t v, uint32_t n) { put128(v, 0, n); }
+  ZX_HD void zeros(uint32_t n)
+  {
+    while (n > 128) { put128(0, 0, 128); n -= 128; }
+    put128(0, 0, n);
+  }
+  ZX_HD void finish()
+  {
+    if (ws.bits) out(ws.idx, ws.buf, true);
+  }
+};
+
 // Counts bits only (the length pass of variable-rate compression).
 struct CountSink {
   static constexpr bool kSpillPastBudget = true;
@@ -1068,6 +1126,28 @@ ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool r
   return budget - bits;
 }
 
+// The coefficients of a block of 4 to 64 values in the form the plane coder
+// reads them: packed 16-bit halves of each 16-plane chunk, and the bit length
+// of their OR (planes above it are empty).
+// Returns the OR of the coefficients.
+template <int P, int SIZE>
+ZX_HD typename Ints<P>::U prepare_ints(const typename Ints<P>::U (&u)[SIZE],
+                                       Halves<SIZE> (&hv)[P / kEncodeChunk], uint32_t &top)
+{
+  typedef typename Ints<P>::U U;
+  U acc = 0;
+  ZX_UNROLL
+  for (int i = 0; i < SIZE; i++) acc |= u[i];
+  top = P == 32 ? 32u - clz32((uint32_t)acc) : 64u - clz64((uint64_t)acc);
+  ZX_UNROLL
+  for (int c = 0; c < P / kEncodeChunk; c++) chunk_halves<P, SIZE>(u, c, hv[c]);
+  return acc;
+}
+
+template <int P, int SIZE, class Sink>
+ZX_HD uint32_t encode_halves(Sink &w, uint32_t maxbits, uint32_t maxprec,
+                             const Halves<SIZE> (&hv)[P / kEncodeChunk], uint32_t top);
+
 template <int P, int SIZE, class Sink>
 ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
                            const typename Ints<P>::U (&u)[SIZE])
@@ -1075,14 +1155,22 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   if constexpr (SIZE == 256) {
     return encode_ints256<P>(w, maxbits, maxprec, u);
   } else {
-  typedef typename Ints<P>::U U;
+    Halves<SIZE> hv[P / kEncodeChunk];
+    uint32_t top;
+    prepare_ints<P, SIZE>(u, hv, top);
+    return encode_halves<P, SIZE>(w, maxbits, maxprec, hv, top);
+  }
+}
+
+// encode_ints and encode_planes (bitplane.rs:539-610) from prepared chunks.
+template <int P, int SIZE, class Sink>
+ZX_HD uint32_t encode_halves(Sink &w, uint32_t maxbits, uint32_t maxprec,
+                             const Halves<SIZE> (&hv)[P / kEncodeChunk], uint32_t top)
+{
+  {
   typedef typename PWord<SIZE>::T PW;
   const uint32_t budget = with_maxbits(maxbits, maxprec, SIZE) ? maxbits : 0xffffffffu;
   const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
-  U acc = 0;
-  ZX_UNROLL
-  for (int i = 0; i < SIZE; i++) acc |= u[i];
-  const uint32_t top = P == 32 ? 32u - clz32((uint32_t)acc) : 64u - clz64((uint64_t)acc);
 
   uint32_t bits = budget;
   uint32_t empty = (uint32_t)P - (top > kmin ? top : kmin);
@@ -1093,9 +1181,6 @@ ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
   uint32_t n = 0;
   constexpr int CH = kEncodeChunk;
   static_assert(CH == 16, "the encoder's chunks are 16 planes");
-  Halves<SIZE> hv[P / CH];
-  ZX_UNROLL
-  for (int c = 0; c < P / CH; c++) chunk_halves<P, SIZE>(u, c, hv[c]);
   ZX_UNROLL
   for (int c = P / CH - 1; c >= 0; c--) {
     const bool need = bits != 0 && kstart > (uint32_t)(CH * c) && kmin <= (uint32_t)(CH * c + CH - 1);
@@ -1686,6 +1771,152 @@ ZX_HD uint32_t encode_reversible(Sink &w, const typename Ints<ST<S>::P>::U (&b)[
       bits = 2 + rev_encode_ints<P, D>(w, q, sat_sub(c.maxbits, 2), c.maxprec);
     }
   }
+  if (bits < c.minbits) {
+    w.zeros(c.minbits - bits);
+    bits = c.minbits;
+  }
+  return bits;
+}
+
+// ---------------------------------------------------------------- prepared blocks
+
+// A block of 4 to 64 values ready for coding, so that a kernel can count its
+// bits and later write them from the same registers (the one-pass
+// variable-rate compress): the header bits, the coefficients' plane chunks
+// and encode_ints' parameters.  Every block code is the header, then (when
+// `coded`) encode_ints(ibits, iprec), then zeros up to min_bits; the fields
+// restate encode_lossy and encode_reversible above case by case.
+template <class S, int D>
+struct PreparedBlock {
+  static constexpr int P = ST<S>::P;
+  Halves<Blk<D>::SIZE> hv[P / kEncodeChunk];
+  uint32_t top;
+  uint64_t hdr;
+  uint32_t hbits;
+  uint32_t ibits;
+  uint32_t iprec;
+  bool coded;
+};
+
+// Sequency order and negabinary (encode/core.rs:12-109), then prepare_ints;
+// returns the OR of the coefficients.
+template <int P, int D>
+ZX_HD typename Ints<P>::U prepare_coeffs(const typename Ints<P>::I (&q)[Blk<D>::SIZE],
+                                         Halves<Blk<D>::SIZE> (&hv)[P / kEncodeChunk], uint32_t &top)
+{
+  typedef typename Ints<P>::U U;
+  U u[Blk<D>::SIZE];
+  ZX_UNROLL
+  for (int j = 0; j < Blk<D>::SIZE; j++) u[j] = ((U)q[perm(D, j)] + Ints<P>::NB) ^ Ints<P>::NB;
+  return prepare_ints<P, Blk<D>::SIZE>(u, hv, top);
+}
+
+// encode_int_block's transform and rounding (before its planes)
+template <int P, int D>
+ZX_HD void prepare_int_block(typename Ints<P>::I (&q)[Blk<D>::SIZE], uint32_t maxprec, int rounding,
+                             Halves<Blk<D>::SIZE> (&hv)[P / kEncodeChunk], uint32_t &top)
+{
+  typedef typename Ints<P>::I I;
+  fwd_xform<D>(q, FwdL());
+  if (rounding == ROUND_FIRST && maxprec < (uint32_t)P) {
+    const I bias = (I)((Ints<P>::NB >> 2) >> maxprec);
+    ZX_UNROLL
+    for (int i = 0; i < Blk<D>::SIZE; i++) q[i] = (maxprec & 1u) ? wadd(q[i], bias) : wsub(q[i], bias);
+  }
+  prepare_coeffs<P, D>(q, hv, top);
+}
+
+template <class S, int D, bool REV>
+ZX_HD void prepare_block(const typename Ints<ST<S>::P>::U (&b)[Blk<D>::SIZE], const Cfg &c,
+                         PreparedBlock<S, D> &pb)
+{
+  constexpr int P = ST<S>::P, SIZE = Blk<D>::SIZE;
+  typedef typename Ints<P>::I I;
+  typedef typename Ints<P>::U U;
+  I q[SIZE];
+  pb.hdr = 0;
+  pb.hbits = 0;
+  pb.coded = true;
+  if constexpr (!REV) {
+    if constexpr (!ST<S>::FLT) {
+      ZX_UNROLL
+      for (int i = 0; i < SIZE; i++) q[i] = (I)b[i];
+      pb.ibits = c.maxbits;
+      pb.iprec = c.maxprec;
+      prepare_int_block<P, D>(q, c.maxprec, c.rounding, pb.hv, pb.top);
+    } else {
+      const int emax = exponent_block<S, SIZE>(b);
+      const uint32_t prec = precision_f(emax, c.maxprec, c.minexp, D, c.tight);
+      const uint32_t e = prec ? (uint32_t)(emax + ST<S>::EBIAS) : 0u;
+      if (e) {
+        const uint32_t hb = ST<S>::EBITS + 1;
+        pb.hdr = 2ull * e + 1;
+        pb.hbits = hb;
+        pb.ibits = sat_sub(c.maxbits, hb);
+        pb.iprec = prec;
+        cast_block<S, SIZE>(b, emax, q);
+        prepare_int_block<P, D>(q, prec, c.rounding, pb.hv, pb.top);
+      } else {
+        pb.hbits = 1;
+        pb.coded = false;
+      }
+    }
+  } else {
+    constexpr uint32_t PB = P == 32 ? 5u : 6u;
+    uint32_t lead = 0, lbits = 0;  // the header before the precision bits
+    uint32_t room = c.maxbits;
+    if constexpr (!ST<S>::FLT) {
+      ZX_UNROLL
+      for (int i = 0; i < SIZE; i++) q[i] = (I)b[i];
+    } else {
+      const int emax = exponent_block<S, SIZE>(b);
+      const uint32_t e = (uint32_t)(emax + ST<S>::EBIAS);
+      const bool tiny = e != 0 && emax < ST<S>::MINCAST;
+      bool rev = !tiny;
+      ZX_UNROLL
+      for (int i = 0; i < SIZE; i++) {
+        bool ex;
+        q[i] = e ? cast_one<S>(b[i], emax, ex) : (I)0;
+        if (!e) ex = b[i] == 0;
+        rev = rev && ex;
+      }
+      if (rev && !e) {
+        pb.hbits = 1;
+        pb.coded = false;
+        return;
+      }
+      if (rev) {
+        lead = 1u | (e << 2);  // 0b01, then the exponent
+        lbits = 2 + ST<S>::EBITS;
+      } else {
+        const U tcmask = ~((U)1 << (P - 1));
+        ZX_UNROLL
+        for (int i = 0; i < SIZE; i++) q[i] = (I)((b[i] >> (P - 1)) ? (b[i] ^ tcmask) : b[i]);
+        lead = 3;  // 0b11
+        lbits = 2;
+      }
+      room = sat_sub(c.maxbits, lbits);
+    }
+    fwd_xform<D>(q, RFwdL());
+    // rev_encode_ints: the precision from the coefficients' lowest set bit
+    const U acc = prepare_coeffs<P, D>(q, pb.hv, pb.top);
+    uint32_t prec = acc ? (uint32_t)P - ctz64((uint64_t)acc) : 0u;
+    prec = umin(prec, c.maxprec);
+    if (prec < 1) prec = 1;
+    pb.hdr = (uint64_t)lead | ((uint64_t)(prec - 1) << lbits);
+    pb.hbits = lbits + PB;
+    pb.ibits = sat_sub(room, PB);
+    pb.iprec = prec;
+  }
+}
+
+// The code of a prepared block through `w`; returns its bits, padding included.
+template <class S, int D, class Sink>
+ZX_HD uint32_t code_block(Sink &w, const PreparedBlock<S, D> &pb, const Cfg &c)
+{
+  w.put(pb.hdr, pb.hbits);
+  uint32_t bits = pb.hbits;
+  if (pb.coded) bits += encode_halves<ST<S>::P, Blk<D>::SIZE>(w, pb.ibits, pb.iprec, pb.hv, pb.top);
   if (bits < c.minbits) {
     w.zeros(c.minbits - bits);
     bits = c.minbits;

@@ -244,4 +244,170 @@ ZX_HD uint64_t fx_dec_block(const uint64_t *lds, const FxPlan &p, const FxGroup 
   }
 }
 
+// ------------------------------------------------------------------ one pass
+
+// One-pass variable-rate compress (the single-array path for 1-3-D fields):
+// a workgroup of T consecutive blocks prepares each block once
+// (prepare_block), counts its bits, scans the counts across the workgroup,
+// finds the bits of all earlier workgroups by a decoupled look-back, and then
+// codes its blocks from the same registers: into LDS at their offsets when
+// the group's word range fits `cap` words, straight to HBM otherwise (the
+// group's words zeroed first, blocks' shared words merged with atomic ORs).
+// Words wholly inside a group's range are stored by that group; the two
+// words it may share with its neighbours (its first, when it starts inside a
+// word, and its last, when it ends inside one) go to `edges`, and a second
+// launch (op_merge) ORs each shared word together from the two groups' edges.
+// No stream word is written twice and none needs clearing beforehand.  This
+// restates the reference's chunked compress (compress.rs:132-236: chunks
+// coded apart, then concatenated by bit-shifted appends) with the offsets
+// found on the device instead of by a serial append.
+struct OnePlan {
+  uint32_t T;          // blocks per workgroup
+  uint32_t cap;        // words a workgroup may stage in LDS
+  uint32_t skew;       // LDS slot of word i: i + (i >> skew)
+  uint32_t lds_words;  // LDS words per workgroup
+};
+
+// Most bits one block of `size` values can take: min_bits, or the header
+// (at most 19 bits: a reversible float's), every coded plane's verbatim bits
+// and one test (planes <= min(max_prec, P)), and at most three bits per
+// coefficient over all planes (its zeros scanned, its value bit and its
+// test), capped by the budget.
+ZX_HD uint32_t block_bound(const Cfg &c, int P, int size)
+{
+  const uint64_t planes = c.maxprec < (uint32_t)P ? c.maxprec : (uint32_t)P;
+  const uint64_t coded = 19 + planes * (uint64_t)(size + 1) + 3ull * size;
+  const uint64_t budget = c.maxbits > 19 ? c.maxbits : 19;
+  const uint64_t most = coded < budget ? coded : budget;
+  return (uint32_t)(most > c.minbits ? most : c.minbits);
+}
+
+// LDS a one-pass workgroup may stage: more costs more than the groups that
+// spill past it (measured on the 384^3 f64 accuracy workload).
+constexpr uint64_t kOpRoomWords = (24 * 1024 / 8 - 2) * 8 / 9;
+
+// The plan for blocks of at most `bound` bits, `mean` bits on average when a
+// previous call on the stream tells (0 when not): the largest workgroup of
+// 256, 128 or 64 blocks whose expected range (1.5 mean blocks, or the
+// largest blocks) fits the room, 64 when none does, staging that range or
+// the room, whichever is less.  `T` and `cap` override when nonzero.
+ZX_HD OnePlan op_plan(uint32_t bound, uint32_t mean, uint32_t T, uint32_t cap)
+{
+  OnePlan p;
+  p.skew = 3;
+  uint64_t e = mean ? (uint64_t)mean * 3 / 2 + 1 : bound;
+  if (e > bound) e = bound;
+  p.T = T;
+  if (!p.T) {
+    p.T = 64;
+    for (uint32_t t = 256; t > 64; t >>= 1)
+      if (((uint64_t)t * e + 63) / 64 + 2 <= kOpRoomWords) {
+        p.T = t;
+        break;
+      }
+  }
+  const uint64_t want = cap ? cap : ((uint64_t)p.T * e + 63) / 64 + 2;
+  p.cap = (uint32_t)(want < kOpRoomWords ? want : kOpRoomWords);
+  p.lds_words = fx_slot(p.cap, p.skew) + 1;
+  return p;
+}
+
+// The range of a group: [b0, b1) in stream bits, its word count and shared edges.
+struct OneRange {
+  uint64_t b0, b1;
+  uint64_t w0;      // first word
+  uint32_t nw;      // words touched
+  bool head, tail;  // first / last word shared with a neighbour (or the stream's prefix)
+};
+
+ZX_HD OneRange op_range(uint64_t b0, uint64_t total)
+{
+  OneRange r;
+  r.b0 = b0;
+  r.b1 = b0 + total;
+  r.w0 = b0 >> 6;
+  const uint64_t nw = ((r.b1 + 63) >> 6) - r.w0;
+  r.nw = nw > 0xffffffffull ? 0xffffffffu : (uint32_t)nw;
+  r.head = (b0 & 63) != 0;
+  r.tail = (r.b1 & 63) != 0;
+  return r;
+}
+
+// staged path, phase 2: a prepared block at bit `rel` of the group's first word
+template <class S, int D>
+ZX_HD void op_stage(uint64_t *lds, const OnePlan &p, uint32_t rel, const PreparedBlock<S, D> &pb, const Cfg &c)
+{
+  LdsSink<true> w;
+  w.init(lds, p.skew, rel);
+  code_block<S, D>(w, pb, c);
+  w.finish();
+}
+
+// staged path, phase 3: word i of the group to the stream, or to the
+// group's edges (edges[0]: first word, edges[1]: last word) when shared
+ZX_HD void op_store(uint64_t *words, uint64_t nwords, const uint64_t *lds, const OnePlan &p,
+                    const OneRange &r, uint32_t i, uint64_t *edges)
+{
+  const uint64_t v = lds[fx_slot(i, p.skew)];
+  const bool first = i == 0 && r.head, last = i == r.nw - 1 && r.tail;
+  if (first) edges[0] = v;
+  if (last) edges[1] = v;
+  if (!first && !last && r.w0 + i < nwords) words[r.w0 + i] = v;
+}
+
+// dense path, phase 2: word i of the group cleared where blocks may share it
+ZX_HD void op_clear(uint64_t *words, uint64_t nwords, const OneRange &r, uint32_t i)
+{
+  const bool first = i == 0 && r.head, last = i == r.nw - 1 && r.tail;
+  if (!first && !last && r.w0 + i < nwords) words[r.w0 + i] = 0;
+}
+
+// dense path, phase 3: a prepared block at stream bit `pos`
+template <class S, int D>
+ZX_HD void op_direct(uint64_t *words, uint64_t nwords, const OneRange &r, uint64_t pos,
+                     const PreparedBlock<S, D> &pb, const Cfg &c, uint64_t *edge)
+{
+  EdgeSink w;
+  w.init(words, nwords, pos, r.head ? r.w0 : ~0ull, r.tail ? r.w0 + r.nw - 1 : ~0ull, edge);
+  code_block<S, D>(w, pb, c);
+  w.finish();
+}
+
+// op_merge for group g of `groups`: end[g] is the bit after its last block
+// in the low 62 bits (the scan's inclusive status word), the first group
+// starting at `start`; edges[2g] and edges[2g + 1] hold its shared first and
+// last words.  A word that groups share is written by the last group that
+// starts inside it: its own first word ORed with the bits every earlier group
+// left in that word (their last words, back to a group that began before the
+// word or at its first bit) and, when the walk reaches the first group, the
+// caller's `prefix` bits below `start`.  A last group that ends inside a word
+// it did not start inside writes that word alone.  (Groups of 64 blocks or
+// more hold at least 64 bits, so the walk takes one step; smaller groups
+// may share a word several ways.)
+ZX_HD void op_merge(uint64_t *words, uint64_t nwords, const uint64_t *end, const uint64_t *edges,
+                    uint64_t g, uint64_t groups, uint64_t start, uint64_t prefix)
+{
+  constexpr uint64_t M = (1ull << 62) - 1;
+  const uint64_t b0 = g ? end[g - 1] & M : start;
+  const uint64_t b1 = end[g] & M;
+  const uint64_t w = b0 >> 6;
+  if ((b0 & 63) && !(g + 1 < groups && (b1 >> 6) == w)) {
+    uint64_t v = edges[2 * g];
+    for (uint64_t h = g;;) {
+      if (h == 0) {
+        v |= prefix;
+        break;
+      }
+      h--;
+      // group h ends at the bit where h + 1 starts, inside word w
+      v |= edges[2 * h + 1];
+      const uint64_t h0 = h ? end[h - 1] & M : start;
+      if (h0 <= 64 * w) break;  // it began before the word or at its first bit
+    }
+    if (w < nwords) words[w] = v;
+  }
+  const uint64_t last = (b1 - 1) >> 6;
+  if (g == groups - 1 && (b1 & 63) && !((b0 & 63) && last == w) && last < nwords) words[last] = edges[2 * g + 1];
+}
+
 }  // namespace zx

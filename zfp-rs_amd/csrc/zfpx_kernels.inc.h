@@ -22,6 +22,7 @@
 #include "zfpx_api.h"
 #include "zfpx_field.h"
 #include "zfpx_fx.h"
+#include "zfpx_scan.h"
 
 namespace zx {
 
@@ -282,6 +283,75 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideE
   }
 }
 
+// One-pass variable-rate compress (zfpx_fx.h, op_*): each workgroup takes a
+// ticket for its group of blocks, prepares and counts its blocks, scans the
+// counts, looks back for the bits of the groups before it, then codes its
+// blocks from the same registers into LDS (or, for a group over the LDS
+// capacity, straight to HBM) and stores its words; the shared edge words go
+// to a_op_edges for k_op_merge.  The staged/direct choice is uniform over the
+// workgroup, so the barriers are too.
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kEncodeWaves<S, REV>, 8))) void k_encode_1p(EncodeArgs a, OnePlan p)
+{
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t group_s;
+  __shared__ uint32_t wave_tot[kThreads / 64];
+  __shared__ uint64_t prefix_s;
+  __shared__ uint64_t edge_s[2];
+  const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t == 0) {
+    group_s = (uint32_t)atomicAdd((unsigned long long *)&a.op_state[0], 1ull);
+    edge_s[0] = 0;
+    edge_s[1] = 0;
+  }
+  expand8_fill();
+  const uint64_t g = group_s;
+  const uint64_t b = g * p.T + t;
+  const bool live = b < a.field.nblocks;
+  PreparedBlock<S, D> pb;
+  uint32_t len = 0;
+  if (live) {
+    typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+    load_block<S, D>(a.field, b, a.vec, v);
+    prepare_block<S, D, REV>(v, a.cfg, pb);
+    CountSink cs{0};
+    len = code_block<S, D>(cs, pb, a.cfg);
+    if (a.lengths) a.lengths[b] = len;
+  }
+  const uint32_t inc = wave_inclusive_sum(len);
+  if (lane == 63) wave_tot[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (uint32_t w = 0; w < p.T / 64; w++) {
+    before += w < wave ? wave_tot[w] : 0u;
+    total += wave_tot[w];
+  }
+  if (wave == 0) {
+    const uint64_t pre = lookback(a.op_state + 2, g, total, a.start, a.op_state + 1);
+    if (lane == 0) prefix_s = pre;
+  }
+  __syncthreads();
+  const OneRange r = op_range(prefix_s, total);
+  const uint32_t off = before + (inc - len);  // the block's first bit within the group
+  if (r.nw <= p.cap) {
+    for (uint32_t i = t; i < r.nw; i += p.T) lds[fx_slot(i, p.skew)] = 0;
+    __syncthreads();
+    if (live) op_stage<S, D>(lds, p, (uint32_t)(r.b0 & 63) + off, pb, a.cfg);
+    __syncthreads();
+    for (uint32_t i = t; i < r.nw; i += p.T) op_store(a.words, a.nwords, lds, p, r, i, edge_s);
+  } else {
+    for (uint32_t i = t; i < r.nw; i += p.T) op_clear(a.words, a.nwords, r, i);
+    __threadfence();
+    __syncthreads();
+    if (live) op_direct<S, D>(a.words, a.nwords, r, r.b0 + off, pb, a.cfg, edge_s);
+  }
+  __syncthreads();
+  if (t == 0) {
+    a.op_edges[2 * g] = edge_s[0];
+    a.op_edges[2 * g + 1] = edge_s[1];
+  }
+}
+
 // ------------------------------------------------------------------ batches
 
 // The array holding a launch's block g: the last j with firsts[j] <= g.
@@ -381,8 +451,15 @@ static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
 {
   if (a.field.nblocks == 0) return hipSuccess;
   if constexpr (D == 4) {
+    if (a.op_state) return hipErrorNotSupported;
     return launch_global_encode_t<S, D, REV>(a, s);
   } else {
+    if (a.op_state) {
+      const OnePlan p = op_plan(block_bound(a.cfg, ST<S>::P, Blk<D>::SIZE), 0, a.op_T, a.vcap);
+      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+      hipLaunchKernelGGL((k_encode_1p<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+      return hipGetLastError();
+    }
     FxPlan p;
     if (!a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p)) {
       const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;

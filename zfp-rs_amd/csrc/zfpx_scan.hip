@@ -3,17 +3,15 @@
 // the batch helpers.
 //
 // The scan is one launch (after a fill that clears its state): a tile of 256
-// threads x 8 lengths takes a ticket, so tiles start in ticket order, scans
-// its lengths within each wave with DPP row shifts and row broadcasts and
-// across its four waves through LDS, and finds the lengths of all earlier
-// tiles by a decoupled look-back (Merrill and Garland, "Single-pass parallel
-// prefix scan with decoupled look-back", NVIDIA TR 2016): it publishes its own
-// total at once, then one wave reads the statuses of the 64 tiles before it in
-// one step, adds the totals down to the nearest tile whose inclusive prefix is
-// known (found with a ballot) and publishes its own inclusive prefix.
+// threads x 8 lengths takes a ticket, scans its lengths within each wave with
+// DPP row shifts and row broadcasts and across its four waves through LDS,
+// and finds the lengths of all earlier tiles by the decoupled look-back of
+// zfpx_scan.h.
 #include <hip/hip_runtime.h>
 
 #include "zfpx_api.h"
+#include "zfpx_fx.h"
+#include "zfpx_scan.h"
 
 namespace {
 
@@ -21,94 +19,8 @@ constexpr int kT = 256;
 constexpr int kItems = 8;
 constexpr uint64_t kTile = (uint64_t)kT * kItems;
 
-// tile status words: the value in the low 62 bits, the kind in the top two
-constexpr uint64_t kAgg = 1ull << 62;   // the tile's own total
-constexpr uint64_t kIncl = 2ull << 62;  // the total of every tile up to and including it
-constexpr uint64_t kValue = (1ull << 62) - 1;
-// a look-back that waits this many rounds gives up and marks the scan
-// failed (state word 1); the tile then takes a prefix no stream reaches
-constexpr uint32_t kSpinLimit = 1u << 24;
-constexpr uint64_t kFailedPrefix = 1ull << 61;
-
-// Inclusive prefix sum of v over the 64 lanes of the wave: Hillis-Steele
-// within each 16-lane row (row_shr 1, 2, 4, 8; a lane with no source in its
-// row adds 0), then row 0's last lane into row 1 and row 2's into row 3
-// (row_bcast:15 on rows 1 and 3), then lane 31 into rows 2 and 3
-// (row_bcast:31).
-__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
-{
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
-  return v;
-}
-
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
-{
-  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-  return v;
-}
-
-// Status words carry their own value and nothing else is read through them,
-// so relaxed atomics at device scope suffice (an acquire or release here
-// would invalidate or write back the caches at every access).
-__device__ __forceinline__ uint64_t load_status(const uint64_t *p)
-{
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_status(uint64_t *p, uint64_t v)
-{
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-}  // namespace
-
-// The exclusive prefix of the tile with ticket `tile` whose own total is
-// `total`, by one wave (all 64 lanes call it): `status` holds one word per
-// tile, cleared before the launch; `failed` is set when the look-back gives
-// up.  Tile 0's prefix is `start`.
-__device__ uint64_t zx_lookback(uint64_t *status, uint64_t tile, uint64_t total, uint64_t start,
-                                uint64_t *failed)
-{
-  const uint32_t lane = threadIdx.x & 63;
-  if (tile == 0) {
-    if (lane == 0) store_status(&status[0], kIncl | ((start + total) & kValue));
-    return start;
-  }
-  if (lane == 0) store_status(&status[tile], kAgg | total);
-  uint64_t prefix = 0;
-  int64_t j = (int64_t)tile - 1;
-  uint32_t spins = 0;
-  for (;;) {
-    const int64_t k = j - (int64_t)lane;
-    const uint64_t st = k >= 0 ? load_status(&status[k]) : kIncl;  // before tile 0: nothing
-    const uint64_t incl = __ballot(st >= kIncl);
-    const uint64_t none = __ballot(st < kAgg);
-    const uint32_t first = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
-    const uint64_t below = first < 64 ? (1ull << first) - 1 : ~0ull;
-    if (none & below) {
-      // a tile between this one and the nearest inclusive prefix has not
-      // published yet: read the window again
-      if (++spins > kSpinLimit) {
-        if (lane == 0) *failed = 1;
-        prefix = kFailedPrefix;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    prefix += wave_sum_u64(lane <= first ? (st & kValue) : 0ull);
-    if (first < 64) break;
-    j -= 64;
-  }
-  if (lane == 0) store_status(&status[tile], kIncl | ((prefix + total) & kValue));
-  return prefix;
-}
-
-namespace {
+using zx::lookback;
+using zx::wave_inclusive_sum;
 
 // state[0]: ticket counter; state[1]: failure flag; state[2 + t]: tile t
 __global__ __launch_bounds__(kT) void k_scan(const uint32_t *len, uint64_t n, uint64_t start,
@@ -137,7 +49,7 @@ __global__ __launch_bounds__(kT) void k_scan(const uint32_t *len, uint64_t n, ui
     total += wave_tot[w];
   }
   if (wave == 0) {
-    const uint64_t p = zx_lookback(state + 2, tile, total, start, state + 1);
+    const uint64_t p = lookback(state + 2, tile, total, start, state + 1);
     if (lane == 0) prefix_s = p;
   }
   __syncthreads();
@@ -147,6 +59,15 @@ __global__ __launch_bounds__(kT) void k_scan(const uint32_t *len, uint64_t n, ui
     o += v[i];
   }
   if (base < n && base + kItems >= n) off[n] = o;  // the thread holding the last length
+}
+
+__global__ __launch_bounds__(kT) void k_op_merge(uint64_t *words, uint64_t nwords, const uint64_t *state,
+                                                 const uint64_t *edges, uint64_t groups, uint64_t start,
+                                                 uint64_t prefix)
+{
+  const uint64_t g = (uint64_t)blockIdx.x * kT + threadIdx.x;
+  if (g >= groups) return;
+  zx::op_merge(words, nwords, state + 2, edges, g, groups, start, prefix);
 }
 
 __global__ void k_fill(uint64_t *w, uint64_t n, uint64_t first)
@@ -241,6 +162,16 @@ hipError_t zx_launch_fill(uint64_t *words, uint64_t n, uint64_t first, hipStream
 {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, words, n, first);
+  return hipGetLastError();
+}
+
+hipError_t zx_launch_op_merge(uint64_t *words, uint64_t nwords, const uint64_t *state,
+                              const uint64_t *edges, uint64_t groups, uint64_t start, uint64_t prefix,
+                              hipStream_t s)
+{
+  if (groups == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_op_merge, dim3((groups + kT - 1) / kT), dim3(kT), 0, s, words, nwords, state, edges,
+                     groups, start, prefix);
   return hipGetLastError();
 }
 
