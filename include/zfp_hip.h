@@ -148,6 +148,12 @@ zfp_bool zfp_hip_field_may_alias(const zfp_field *field);
  * the armed failure has been used or none was armed. */
 long zfp_hip_fail_after(long n);
 
+/* Why the calling thread's last failed call (0 or NULL returned) failed inside
+ * this library: the source line and the HIP error or reservation, as text;
+ * empty before any failure.  No counterpart in the C ABI, whose calls return
+ * 0 for every failure. */
+const char *zfp_hip_last_error(void);
+
 /* Device the library runs on (the current HIP device when first used). */
 int zfp_hip_device(void);
 

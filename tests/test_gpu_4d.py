@@ -127,7 +127,8 @@ def test_4d_block_entry_points(ztype):
         assert np.array_equal(buf[: (obits + 63) // 64], ow[: (obits + 63) // 64])
         Z.lib().zfp_stream_rewind(s.zs)
         out = np.zeros_like(blk)
-        assert fn(f"zfp_decode_block_{name}_4", [])(s.zs, out.ctypes.data) == obits
+        got = fn(f"zfp_decode_block_{name}_4", [])(s.zs, out.ctypes.data)
+        assert got == obits, (got, obits, Z.last_error())
         st.rewind()
         ref = np.zeros_like(blk)
         O.decode_block(st, oc, ztype, 4, ref.reshape(-1), 0, [1, 4, 16, 64], [4] * 4)

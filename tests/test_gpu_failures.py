@@ -83,6 +83,7 @@ def test_failed_compress_leaves_the_stream(name, mk, device):
             break
         met_any = True
         assert n == 0, (name, k)
+        assert "injected reservation failure" in Z.last_error(), (name, k)
         assert s.wtell() == pos, (name, k)
         assert np.array_equal(words_of(buf), before), (name, k)
         # the next call on the same stream codes normally

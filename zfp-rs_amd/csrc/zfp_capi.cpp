@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -51,6 +52,16 @@ bool is_device_ptr(const void *p)
 }
 
 // Grow-only device buffers shared by the codec calls of this process.
+// The reason the calling thread's last call failed inside the library
+// (zfp_hip_last_error): the source line and the HIP error or reservation
+// that failed.
+thread_local char g_last_error[96];
+
+void note_failure(int line, const char *what)
+{
+  snprintf(g_last_error, sizeof g_last_error, "zfp_capi.cpp:%d: %s", line, what);
+}
+
 // Injected reservation failures for tests (zfp_hip_fail_after): counts the
 // reservations of workspace, staging and pinned memory from the moment it is
 // armed, and makes the n-th fail as an out-of-memory would, once.
@@ -82,13 +93,17 @@ struct Workspace {
 
   static bool grow(void *&p, size_t &cap, size_t need)
   {
-    if (injected_failure()) return false;
+    if (injected_failure()) {
+      note_failure(__LINE__, "injected reservation failure");
+      return false;
+    }
     if (need <= cap) return true;
     const size_t n = std::max(need, cap + cap / 2);
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
     if (hipMalloc(&p, n) != hipSuccess) {
+      note_failure(__LINE__, "workspace hipMalloc");
       (void)hipGetLastError();
       p = nullptr;
       return false;
@@ -837,12 +852,14 @@ bool fixed_layout(const zx::Cfg &c, const Plan &p)
   return c.minbits == c.maxbits && block_bits(c, p.type, p.dims) == c.maxbits && c.maxbits > 0;
 }
 
-#define ZCHECK(x)                     \
-  do {                                \
-    if ((x) != hipSuccess) {          \
-      (void)hipGetLastError();        \
-      return false;                   \
-    }                                 \
+#define ZCHECK(x)                                        \
+  do {                                                   \
+    const hipError_t zcheck_e = (x);                     \
+    if (zcheck_e != hipSuccess) {                        \
+      note_failure(__LINE__, hipGetErrorName(zcheck_e)); \
+      (void)hipGetLastError();                           \
+      return false;                                      \
+    }                                                    \
   } while (0)
 
 // Device pointer to element [0,0,0] of the field: the caller's own, or the
@@ -1080,7 +1097,10 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
 
 bool grow_host(void *&p, size_t &cap, size_t need)
 {
-  if (injected_failure()) return false;
+  if (injected_failure()) {
+    note_failure(__LINE__, "injected reservation failure");
+    return false;
+  }
   if (need <= cap) return true;
   const size_t n = std::max(need, cap + cap / 2);
   if (p) (void)hipHostFree(p);
@@ -1461,7 +1481,10 @@ BatchBuffers &batch_buffers()
 
 bool grow_pinned(void *&p, size_t &cap, size_t need)
 {
-  if (injected_failure()) return false;
+  if (injected_failure()) {
+    note_failure(__LINE__, "injected reservation failure");
+    return false;
+  }
   if (need <= cap) return true;
   const size_t n = std::max(need, cap + cap / 2);
   if (p) (void)hipHostFree(p);
@@ -3070,6 +3093,8 @@ size_t zfp_hip_batch_wait(zfp_hip_batch *P, size_t *sizes)
   }
   return batch_finish(P->S, P->D, P->zs.data(), sizes);
 }
+
+const char *zfp_hip_last_error(void) { return g_last_error; }
 
 long zfp_hip_fail_after(long n)
 {

@@ -655,6 +655,7 @@ struct EdgeSink {
 // Counts bits only (the length pass of variable-rate compression).
 struct CountSink {
   static constexpr bool kSpillPastBudget = true;
+  static constexpr bool kCountOnly = true;  // plane codes need only their lengths
   uint64_t n;
   ZX_HD void put(uint64_t, uint32_t k) { n += k; }
   ZX_HD void put128(uint64_t, uint64_t, uint32_t k) { n += k; }
@@ -940,6 +941,28 @@ ZX_HD void expand_ones(uint64_t u, uint64_t &lo, uint64_t &hi)
 // significant (bitplane.rs:560-605): the n low bits verbatim, then the group
 // tests over the rest.  Returns the code's length; the code is (lo, hi) and
 // nn is the new n.
+// plane_code's length and new significant count only (the bits of the code
+// are not built): a counting pass need not expand the plane.
+template <int SIZE>
+ZX_HD uint32_t plane_len(uint64_t x, uint32_t n, uint32_t &nn)
+{
+  nn = n;
+  if (n >= (uint32_t)SIZE) return SIZE;
+  const uint64_t rest = x >> n;
+  if (rest == 0) return n + 1;
+  const uint32_t q = 63u - clz64(rest);
+  const uint32_t tb = q + popc64(rest) - 1;
+  const bool implied = n + q == (uint32_t)SIZE - 1;
+  nn = n + q + 1;
+  return n + (implied ? tb + 1 : tb + 3);
+}
+
+// Whether a sink only counts bits (CountSink::kCountOnly).
+template <class Sink, class = void> struct sink_counts { static constexpr bool value = false; };
+template <class Sink> struct sink_counts<Sink, decltype((void)Sink::kCountOnly)> {
+  static constexpr bool value = Sink::kCountOnly;
+};
+
 template <int SIZE>
 ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, uint32_t &nn)
 {
@@ -1191,9 +1214,11 @@ ZX_HD uint32_t encode_halves(Sink &w, uint32_t maxbits, uint32_t maxprec,
     for (int i = CH - 1; i >= 0; i--) {
       const uint32_t k = (uint32_t)(CH * c + i);
       if (bits != 0 && k < kstart && k >= kmin) {
-        uint64_t clo, chi;
+        uint64_t clo = 0, chi = 0;
         uint32_t nn;
-        const uint32_t len = plane_code<SIZE>((uint64_t)ch[i], n, clo, chi, nn);
+        uint32_t len;
+        if constexpr (sink_counts<Sink>::value) len = plane_len<SIZE>((uint64_t)ch[i], n, nn);
+        else len = plane_code<SIZE>((uint64_t)ch[i], n, clo, chi, nn);
         const uint32_t take = umin(len, bits);
         if constexpr (Sink::kSpillPastBudget)
           w.put128(clo, chi, take);

@@ -126,6 +126,7 @@ SIGNATURES = {
     "zfp_stream_kernel_ms": (_D, [_SP, P(_D)]),
     "zfp_hip_device": (ctypes.c_int, []),
     "zfp_hip_fail_after": (ctypes.c_long, [ctypes.c_long]),
+    "zfp_hip_last_error": (ctypes.c_char_p, []),
     "zfp_hip_field_may_alias": (_B, [_FP]),
     "zfp_hip_compress_noalloc": (_SZ, [_SP, _FP]),
     "zfp_hip_noalloc_max_blocks": (_U64, []),
@@ -315,7 +316,8 @@ class Stream:
         f = make_field(arr)
         n = lib().zfp_compress(self.zs, ctypes.byref(f))
         if n == 0:
-            raise BufferTooSmall("compress failed (buffer too small or invalid field)")
+            raise BufferTooSmall("compress failed (buffer too small or invalid field; "
+                                 f"last library failure: {last_error()})")
         return n
 
     def decompress(self, arr):
@@ -323,7 +325,8 @@ class Stream:
         f = make_field(arr)
         n = lib().zfp_decompress(self.zs, ctypes.byref(f))
         if n == 0:
-            raise Truncated("decompress failed (truncated stream or invalid field)")
+            raise Truncated("decompress failed (truncated stream or invalid field; "
+                            f"last library failure: {last_error()})")
         return n
 
     def rewind(self):
@@ -453,6 +456,11 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+def last_error():
+    """zfp_hip_last_error: why this thread's last failed call failed, or ''."""
+    return lib().zfp_hip_last_error().decode()
 
 
 def compress_batch(streams, arrays, flags=0):
