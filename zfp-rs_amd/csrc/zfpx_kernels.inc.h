@@ -57,13 +57,25 @@ __device__ __forceinline__ void expand8_fill()
   __syncthreads();
 }
 
-// Fills the workgroup's compress8 table (zfpx_block.h); every kernel that
-// decodes calls it first, with all of its threads, before any thread leaves.
+// Fills the workgroup's compress8 table (zfpx_block.h), which the decoders of
+// 3-D blocks read; every kernel that decodes them calls it first, with all of
+// its threads, before any thread leaves.  Other dimensionalities skip it.
+template <int D>
+__device__ __forceinline__ void compress8_load()
+{
+  if constexpr (D == 3)
+    for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x)
+      ((uint32_t *)zx_compress8_lds)[i] = ((const uint32_t *)kCompress8Dev.v)[i];
+}
+
+// compress8_load and the barrier after it (3-D only)
+template <int D>
 __device__ __forceinline__ void compress8_fill()
 {
-  for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x)
-    ((uint32_t *)zx_compress8_lds)[i] = ((const uint32_t *)kCompress8Dev.v)[i];
-  __syncthreads();
+  if constexpr (D == 3) {
+    compress8_load<D>();
+    __syncthreads();
+  }
 }
 
 // Rows of four contiguous values are loaded and stored as 16-byte vectors
@@ -163,7 +175,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideE
 template <class S, int D, bool REV>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_decode(DecodeArgs a)
 {
-  compress8_fill();
+  compress8_fill<D>();
   const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= (a.count ? a.count : a.field.nblocks)) return;
   const uint64_t b = a.first + i;
@@ -214,7 +226,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecod
   const uint32_t t = threadIdx.x;
   const FxGroup q = fx_group(blockIdx.x, p.T, a.field.nblocks, a.start, a.fixed);
   for (uint32_t i = t; i < q.nw + 2; i += p.T) fx_dec_stage(lds, a.words, a.nwords, p, q, i);
-  compress8_fill();
+  compress8_load<D>();
+  __syncthreads();
   if (t >= q.nb) return;
   typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
   const uint64_t loaded = fx_dec_block<S, D, REV, FLAT>(lds, p, q, t, a.fixed, v, a.cfg);
@@ -398,7 +411,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideE
 template <class S, int D, bool REV>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_batch_decode(BatchArgs a)
 {
-  compress8_fill();
+  compress8_fill<D>();
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.nblocks) return;
   const uint32_t j = job_of(a.firsts, a.njobs, g);
