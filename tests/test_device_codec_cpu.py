@@ -214,6 +214,38 @@ def test_random_streams_decode_as_the_oracle(dims, ztype):
                     assert np.array_equal(got.view(u), ref.view(u))
 
 
+@pytest.mark.parametrize("ztype", TYPES)
+def test_many_3d_blocks_near_the_last_coefficient(ztype):
+    """3-D blocks whose planes are parsed with few coefficients left (the
+    decoder's 32-bit window, taken when every block of a wave has at most 16
+    coefficients left, beside the 64-bit one) decode as the oracle decodes
+    them: smooth fields at fixed rates where most blocks end with all 64
+    coefficients significant, and arbitrary streams (budget cuts inside runs,
+    runs reaching the last coefficient) at several bit densities."""
+    rng = np.random.default_rng(300 + ztype)
+    shape = (16, 20, 24)
+    dt = O.DTYPES[ztype]
+    u = np.uint32 if np.dtype(dt).itemsize == 4 else np.uint64
+    nblocks = 4 * 5 * 6
+    for rate in (4, 8, 12, 16, 24, 31):
+        for rnd in (O.ROUND_NEVER, O.ROUND_LAST):
+            base = O.fixed_rate(rate, ztype, 3)
+            c = O.config(base.minbits, base.maxbits, base.maxprec, base.minexp, rnd, False)
+            arr = data_for(ztype, shape, "smooth", rate)
+            n, err, ow = O.compress(arr, c)
+            assert err == 0
+            ref, _, rerr = O.decompress(ow, shape, dt, c)
+            assert rerr == 0
+            for lds_T in (0, 64):
+                got, _ = H.decompress(shape, dt, ztype, c, ow, 0, c.maxbits, None, False, lds_T)
+                assert np.array_equal(got.view(u), ref.view(u)), (rate, rnd, lds_T)
+            for density in (0.6, 0.5, 0.3, 0.1):
+                words = random_words(rng, (nblocks * c.maxbits + 63) // 64 + 8, density)
+                ref, _, rerr = O.decompress(words, shape, dt, c)
+                got, _ = H.decompress(shape, dt, ztype, c, words, 0, c.maxbits, None, False, 64)
+                assert np.array_equal(got.view(u), ref.view(u)), (rate, rnd, density)
+
+
 @pytest.mark.parametrize("dims", [1, 2, 3, 4])
 @pytest.mark.parametrize("ztype", TYPES)
 def test_length_reader_matches_the_oracle(dims, ztype):

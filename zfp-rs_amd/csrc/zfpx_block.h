@@ -60,6 +60,8 @@ ZX_HD uint32_t ctz64(uint64_t x) { return x ? (uint32_t)__builtin_ctzll(x) : 64u
 ZX_HD uint32_t clz32(uint32_t x) { return x ? (uint32_t)__builtin_clz(x) : 32u; }
 ZX_HD uint32_t clz64(uint64_t x) { return x ? (uint32_t)__builtin_clzll(x) : 64u; }
 ZX_HD uint64_t lowmask(uint32_t n) { return n < 64 ? (1ull << n) - 1 : ~0ull; }
+ZX_HD uint32_t ctz32(uint32_t x) { return x ? (uint32_t)__builtin_ctz(x) : 32u; }
+ZX_HD uint32_t lowmask32(uint32_t n) { return n < 32 ? (1u << n) - 1 : ~0u; }
 ZX_HD uint32_t popc32(uint32_t x) { return (uint32_t)__builtin_popcount(x); }
 ZX_HD uint32_t popc64(uint64_t x) { return (uint32_t)__builtin_popcountll(x); }
 ZX_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
@@ -368,7 +370,9 @@ ZX_HD void chunk_halves(const typename Ints<P>::U (&u)[SIZE], int c, Halves<SIZE
     }
     ZX_UNROLL
     for (int i = 0; i < 16; i++) {
-      hv.o[g][i] = hi16 ? (a[i] >> 16) | (a[i + 16] & 0xffff0000u) : (a[i] & 0x0000ffffu) | (a[i + 16] << 16);
+      // the 16-bit halves of two words in one byte permute: low halves
+      // {a[i].b0, a[i].b1, a[i+16].b0, a[i+16].b1}, high halves likewise
+      hv.o[g][i] = perm_bytes(a[i + 16], a[i], hi16 ? 0x07060302u : 0x05040100u);
 #if defined(__HIP_DEVICE_COMPILE__)
       // computed here, not sunk into the chunk's branch (where the
       // coefficients would have to stay live until then)
@@ -687,6 +691,16 @@ struct StagedWords {
     lo = q[0];
     hi = q[1];
   }
+  // FLAT: words i, i + 1 and i + 2 for the word i holding bit p, from one
+  // base (the third may be a word past the block's pad copy: a neighbour's
+  // word or a spare slot, whose bits the block's budget never lets a reader use)
+  ZX_HD void triple_at(uint32_t p, uint64_t &w0, uint64_t &w1, uint64_t &w2) const
+  {
+    const uint64_t *q = w + (p >> 6);
+    w0 = q[0];
+    w1 = q[1];
+    w2 = q[2];
+  }
 };
 
 // Reads a block's bits: a buffer of `bits` bits plus the next two words (the
@@ -737,6 +751,13 @@ struct Source {
     consume(n);
     return v & lowmask(n);
   }
+  // read(n), and the 64 bits that follow in `ahead`
+  ZX_HD uint64_t read_peek(uint32_t n, uint64_t &ahead)
+  {
+    const uint64_t v = read(n);
+    ahead = peek();
+    return v;
+  }
   ZX_HD uint64_t pos() const { return idx * 64 - bits; }
   ZX_HD uint64_t words_loaded() const { return idx; }
 };
@@ -767,6 +788,28 @@ struct Source<StagedWords<FLAT>> {
     const uint64_t v = peek();
     consume(n);
     return v & lowmask(n);
+  }
+  // read(n) (n <= 64), and the 64 bits that follow in `ahead`: over a FLAT
+  // block both come from one load of the three words under them, so a
+  // plane's verbatim bits and its first test window cost one LDS round trip
+  ZX_HD uint64_t read_peek(uint32_t n, uint64_t &ahead)
+  {
+    if constexpr (FLAT) {
+      const uint32_t s = p & 63;
+      uint64_t w0, w1, w2;
+      mem.triple_at(p, w0, w1, w2);
+      const uint64_t v = (w0 >> s) | ((w1 << 1) << (63 - s));
+      const uint32_t t = s + n;  // 0 .. 127
+      const uint32_t u = t & 63;
+      const uint64_t a = t < 64 ? w0 : w1, b = t < 64 ? w1 : w2;
+      ahead = (a >> u) | ((b << 1) << (63 - u));
+      p += n;
+      return v & lowmask(n);
+    } else {
+      const uint64_t v = read(n);
+      ahead = peek();
+      return v;
+    }
   }
   ZX_HD uint64_t pos() const { return p; }
   ZX_HD uint64_t words_loaded() const { return (p + 63) >> 6; }
@@ -884,6 +927,22 @@ ZX_HD uint64_t compress_ones(uint64_t v, uint32_t nbits)
   return out;
 }
 
+// compress_ones of a value below 2^32: the four byte lookups only.
+ZX_HD uint32_t compress_ones32(uint32_t v)
+{
+  uint32_t t[4];
+  ZX_UNROLL
+  for (int m = 0; m < 4; m++) t[m] = compress8_of((v >> (8 * m)) & 0xffu);
+  uint32_t out = t[0];
+  uint32_t cnt = popc32(v & 0xffu);
+  ZX_UNROLL
+  for (int m = 1; m < 4; m++) {
+    out |= t[m] << (8 * m - cnt);
+    cnt += popc32((v >> (8 * m)) & 0xffu);
+  }
+  return out;
+}
+
 // The one-bits of u (SIZE bits at most), the j-th from the bottom moved up
 // by j, as a 128-bit value (lo, hi).  Bytes are expanded through the table
 // and placed 8m + (one-bits below byte m) up; a byte range that is zero on
@@ -984,14 +1043,19 @@ ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, 
   expand_ones<SIZE>(rest, elo, ehi);
   const uint32_t q = 63u - clz64(rest);      // the last one-bit
   const uint32_t tb = q + popc64(rest) - 1;  // E's top bit
-  const uint64_t t = 1ull << (tb & 63);
-  const uint64_t tlo = tb < 64 ? t : 0ull, thi = tb < 64 ? 0ull : t;
-  const uint64_t plo = elo ^ tlo, phi = ehi ^ thi;
   // a one-bit at the last position is implied, with no test after it
   const bool implied = n + q == (uint32_t)SIZE - 1;
-  const uint64_t olo = implied ? plo : elo, ohi = implied ? phi : ehi;
-  const uint64_t glo = 1ull | (olo << 1) | (plo << 2);
-  const uint64_t ghi = (ohi << 1) | (olo >> 63) | (phi << 2) | (plo >> 62);
+  // E's one-bits are at least two apart, so E << 1 and E << 2 share no bit
+  // and 1 | 6E is 1 | E << 1 | E << 2: every value bit with a positive test
+  // after it.  The last one-bit's test is negative (bit tb + 2 cleared) or,
+  // when implied, it has neither value nor test (bits tb + 1 and up cleared).
+  const uint64_t slo = (elo << 2) | (elo << 1);
+  const uint64_t shi = (ehi << 2) | (ehi << 1) | (uint64_t)(((uint32_t)(elo >> 32) >> 30) |
+                                                            ((uint32_t)(elo >> 32) >> 31));
+  const uint32_t keep = implied ? tb + 1 : tb + 2;  // bits of 1 | 6E kept, 1 .. 127
+  // low (2 << (k - 1)) - 1 for k = min(keep, 64); the high word's when keep > 64
+  const uint64_t glo = (slo | 1ull) & ((2ull << (umin(keep, 64u) - 1)) - 1);
+  const uint64_t ghi = keep > 64 ? shi & ((2ull << (keep - 65)) - 1) : 0ull;
   const uint32_t glen = implied ? tb + 1 : tb + 3;
   clo |= glo << n;
   chi = (ghi << n) | ((glo >> 1) >> (63 - n));
@@ -1277,19 +1341,78 @@ ZX_HD void plane_steps(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &o
   }
 }
 
+// plane_tests for a wave whose lanes all have at most 16 coefficients left
+// (n >= SIZE - 16): each positive test is followed by at most two bits per
+// remaining coefficient, the last one implied, so the tests after the first
+// take at most 30 bits and one 32-bit window ends every lane's plane; the
+// same parse in 32-bit arithmetic.
 template <int SIZE, class Src>
-ZX_HD void plane_tests(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &one, bool go)
+ZX_HD void plane_tests_narrow(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &one, bool go,
+                              uint64_t ahead)
+{
+  constexpr uint32_t ODD = 0xaaaaaaaau;
+  constexpr uint32_t W = 31;  // window bits after the first test
+  one = false;
+  if (!go) return;
+  const uint32_t w = (uint32_t)ahead;
+  if (!(w & 1u)) {  // negative first test
+    r.consume(1);
+    bits -= 1;
+    return;
+  }
+  const uint32_t y = w >> 1;
+  const uint32_t z = (y + (y & ~(y << 1) & ODD)) ^ ODD;
+  const uint32_t val = y & z;
+  const uint32_t term = ~y & (y << 1) & z & lowmask32(W);
+  const bool done = term != 0;
+  const uint32_t e = ctz32(term);
+  const uint32_t c = done ? e + 1 : W - ((val >> (W - 1)) & 1u);
+  const uint32_t vr = done ? e : c;
+  const uint32_t vals = val & lowmask32(vr);
+  const uint32_t xn = compress_ones32(vals);
+  const uint32_t coef = done ? c - popc32(vals) : c - popc32(val & lowmask32(c - 1));
+  const uint32_t q = (uint32_t)SIZE - 1 - n;
+  const bool size_hit = n + coef >= (uint32_t)SIZE;
+  const uint32_t len = size_hit ? q + popc32(xn & lowmask32(q)) : c;
+  const uint32_t yb = bits - 1;
+  if (len > yb) {
+    const uint32_t nt = yb ? yb - popc32(val & lowmask32(yb - 1)) : 0u;
+    const bool lastv = yb != 0 && ((val >> (yb - 1)) & 1u);
+    x |= (uint64_t)(xn & lowmask32(nt)) << n;
+    n += nt;
+    if (!lastv) x |= 1ull << n++;
+    r.consume(1 + yb);
+    bits = 0;
+    one = true;
+  } else if (size_hit) {
+    x |= ((uint64_t)(xn & lowmask32(q)) << n) | (1ull << (SIZE - 1));
+    n = SIZE;
+    r.consume(1 + len);
+    bits -= 1 + len;
+    one = true;
+  } else {
+    // the window holds the whole plane: the negative test ends it
+    x |= (uint64_t)xn << n;
+    n += coef;
+    r.consume(1 + c);
+    bits -= 1 + c;
+  }
+}
+
+template <int SIZE, class Src>
+ZX_HD void plane_tests(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &one, bool go,
+                       uint64_t ahead)
 {
   if constexpr (SIZE < 64) {
     plane_steps<SIZE>(r, n, bits, x, one, go);
     return;
   }
   constexpr uint64_t ODD = 0xaaaaaaaaaaaaaaaaull;
-  uint32_t s = 1;  // the first window starts with the first test bit
+  uint32_t s = 1;  // the first window (`ahead`) starts with the first test bit
   one = false;
   while (wave_any(go)) {
     if (go) {
-      const uint64_t w = r.peek();
+      const uint64_t w = s ? ahead : r.peek();
       if (s && !(w & 1u)) {  // negative first test
         r.consume(1);
         bits -= 1;
@@ -1387,16 +1510,24 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
       k -= live ? 1u : 0u;  // == CH*c + i when live
       m = live ? 0u : m;
       act = live && k >= kmin;
-      uint64_t x = 0;
+      uint64_t x = 0, ahead = 0;
       bool tested = false;
       if (act) {
         m = umin(n, bits);
         bits -= m;
-        x = r.read(m);
+        if constexpr (SIZE == 64) x = r.read_peek(m, ahead);
+        else x = r.read(m);
         tested = bits != 0 && n < (uint32_t)SIZE;
       }
       bool one;
-      plane_tests<SIZE>(r, n, bits, x, one, tested);
+      if constexpr (SIZE == 64) {
+        if (wave_all(!tested || n >= (uint32_t)SIZE - 16))
+          plane_tests_narrow<SIZE>(r, n, bits, x, one, tested, ahead);
+        else
+          plane_tests<SIZE>(r, n, bits, x, one, tested, ahead);
+      } else {
+        plane_tests<SIZE>(r, n, bits, x, one, tested, 0);
+      }
       // round_state's m: the count after a plane cut short, or all
       if (tested) m = one ? n : (uint32_t)SIZE;
       pl[i] = (PW)x;
@@ -1486,12 +1617,24 @@ ZX_HD int exponent_block(const typename Ints<ST<S>::P>::U (&b)[SIZE], bool &spec
   constexpr int P = ST<S>::P, EB = ST<S>::EBITS, MANT = ST<S>::MANT;
   const U absmask = ~((U)1 << (P - 1));
   const U inf = (U)(((U)1 << EB) - 1) << MANT;
-  U mx = 0;
+  // The largest magnitude's bits without masking each value: as signed
+  // integers the non-negative values order by magnitude and exceed every
+  // negative one, and as unsigned integers the negative values order by
+  // magnitude and exceed every non-negative one; so the signed maximum is
+  // the largest non-negative value (when there is one) and the unsigned
+  // maximum, less its sign bit, the largest negative magnitude (when there
+  // is one, else the largest non-negative value again).
+  typedef typename Ints<P>::I I;
+  I smx = (I)b[0];
+  U umx = b[0];
   ZX_UNROLL
-  for (int i = 0; i < SIZE; i++) {
-    const U a = b[i] & absmask;
-    mx = a > mx ? a : mx;
+  for (int i = 1; i < SIZE; i++) {
+    smx = (I)b[i] > smx ? (I)b[i] : smx;
+    umx = b[i] > umx ? b[i] : umx;
   }
+  const U pmx = smx < 0 ? (U)0 : (U)smx;
+  const U nmx = umx & absmask;
+  U mx = pmx > nmx ? pmx : nmx;
   special = mx >= inf;
   if (special) {
     // NaNs take no part in the maximum
@@ -1615,28 +1758,56 @@ template <> struct Bits<int64_t> {
   ZX_HD static int64_t make(uint64_t u) { return (int64_t)u; }
 };
 
+// float -> int32 conversion with the device instruction's defined results:
+// truncation, saturation at the int32 limits, 0 for NaN (v_cvt_i32_f32).
+ZX_HD int32_t cvt_i32_sat(float v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  int32_t r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+#else
+  if (v != v) return 0;
+  if (v >= 0x1p31f) return INT32_MAX;
+  if (v < -0x1p31f) return INT32_MIN;
+  return (int32_t)v;
+#endif
+}
+
 // fwd_cast of a whole block (encode/core.rs:323-423): q = trunc(x * 2^(P-2-emax)),
 // and the minimum integer wherever the scaled value is not within +-2^(P-1)
 // (NaN, infinity, overflow), as x86's conversion gives.  The scaling is one
 // ldexp (v_ldexp on the device), which is exact wherever the result is at
 // least one in magnitude, whatever emax (subnormal inputs included); smaller
 // results truncate to zero whatever their rounding, so ldexp and truncation
-// give the exact result.
+// give the exact result.  Every finite value is below 2^(emax+1) in
+// magnitude, so only a block holding an infinity or a NaN (`special`, from
+// exponent_block) can scale out of range.  32-bit values convert with
+// saturation; in such a block NaNs are first made infinities, and the
+// positive limit (which no value within range reaches: the largest float
+// below 2^31 is 2^31 - 128) then becomes the minimum.
 template <class S, int SIZE>
-ZX_HD void cast_block(const typename Ints<ST<S>::P>::U (&b)[SIZE], int emax,
+ZX_HD void cast_block(const typename Ints<ST<S>::P>::U (&b)[SIZE], int emax, bool special,
                       typename Ints<ST<S>::P>::I (&q)[SIZE])
 {
   typedef typename Ints<ST<S>::P>::I I;
+  typedef typename Ints<ST<S>::P>::U U;
   constexpr int P = ST<S>::P;
-  const I imin = (I)((typename Ints<P>::U)1 << (P - 1));
+  const I imin = (I)((U)1 << (P - 1));
   const int k = P - 2 - emax;
   if constexpr (P == 32) {
+    U x[SIZE];
     ZX_UNROLL
-    for (int i = 0; i < SIZE; i++) {
-      const float v = ldexpf(u2f((uint32_t)b[i]), k);
-      const bool in = fabsf(v) < 0x1p31f;
-      q[i] = in ? (I)(in ? v : 0.0f) : imin;
+    for (int i = 0; i < SIZE; i++) x[i] = b[i];
+    if (special) {
+      ZX_UNROLL
+      for (int i = 0; i < SIZE; i++) {
+        const float v = ldexpf(u2f((uint32_t)x[i]), k);
+        x[i] = fabsf(v) < 0x1p31f ? x[i] : 0xff800000u;
+      }
     }
+    ZX_UNROLL
+    for (int i = 0; i < SIZE; i++) q[i] = (I)cvt_i32_sat(ldexpf(u2f((uint32_t)x[i]), k));
   } else {
     ZX_UNROLL
     for (int i = 0; i < SIZE; i++) {
@@ -1663,13 +1834,14 @@ ZX_HD uint32_t encode_lossy(Sink &w, const typename Ints<ST<S>::P>::U (&b)[Blk<D
     for (int i = 0; i < SIZE; i++) q[i] = (I)b[i];
     return encode_int_block<P, D>(w, q, c.minbits, c.maxbits, c.maxprec, c.rounding);
   } else {
-    const int emax = exponent_block<S, SIZE>(b);
+    bool special;
+    const int emax = exponent_block<S, SIZE>(b, special);
     const uint32_t prec = precision_f(emax, c.maxprec, c.minexp, D, c.tight);
     const uint32_t e = prec ? (uint32_t)(emax + ST<S>::EBIAS) : 0u;
     if (e) {
       const uint32_t hb = ST<S>::EBITS + 1;
       w.put(2ull * e + 1, hb);
-      cast_block<S, SIZE>(b, emax, q);
+      cast_block<S, SIZE>(b, emax, special, q);
       return hb + encode_int_block<P, D>(w, q, sat_sub(c.minbits, hb), sat_sub(c.maxbits, hb), prec,
                                          c.rounding);
     }
@@ -1870,7 +2042,8 @@ ZX_HD void prepare_block(const typename Ints<ST<S>::P>::U (&b)[Blk<D>::SIZE], co
       pb.iprec = c.maxprec;
       prepare_int_block<P, D>(q, c.maxprec, c.rounding, pb.hv, pb.top);
     } else {
-      const int emax = exponent_block<S, SIZE>(b);
+      bool special;
+      const int emax = exponent_block<S, SIZE>(b, special);
       const uint32_t prec = precision_f(emax, c.maxprec, c.minexp, D, c.tight);
       const uint32_t e = prec ? (uint32_t)(emax + ST<S>::EBIAS) : 0u;
       if (e) {
@@ -1879,7 +2052,7 @@ ZX_HD void prepare_block(const typename Ints<ST<S>::P>::U (&b)[Blk<D>::SIZE], co
         pb.hbits = hb;
         pb.ibits = sat_sub(c.maxbits, hb);
         pb.iprec = prec;
-        cast_block<S, SIZE>(b, emax, q);
+        cast_block<S, SIZE>(b, emax, special, q);
         prepare_int_block<P, D>(q, prec, c.rounding, pb.hv, pb.top);
       } else {
         pb.hbits = 1;

@@ -48,6 +48,25 @@ ZX_HD uint32_t fx_log2(uint32_t v)
   return s;
 }
 
+// Stream words are read (decode staging) or written (encode) once per call:
+// nontemporal accesses on the device, plain ones in the CPU build.
+ZX_HD uint64_t fx_load_once(const uint64_t *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+ZX_HD void fx_store_once(uint64_t *p, uint64_t v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // LDS a workgroup of the path may use.
 constexpr uint64_t kFxMaxLdsBytes = 64 * 1024;
 
@@ -58,7 +77,11 @@ ZX_HD bool fx_plan(uint32_t fixed, uint64_t start, bool decode, uint32_t T, FxPl
 {
   if (fixed == 0 || fixed > 64 * 64 || T > 256) return false;
   const uint32_t W = (fixed + 63) / 64;
-  p.T = T ? T : W <= 16 ? 256u : W <= 32 ? 128u : 64u;
+  // decode: one wave per workgroup (its waves share nothing but the staging,
+  // and a one-wave workgroup takes any free wave slot, which keeps more waves
+  // resident: 2-3 % faster than 256 on 512^3 f32 at rate 8); encode: 256
+  // blocks, or fewer for large blocks
+  p.T = T ? T : decode ? 64u : W <= 16 ? 256u : W <= 32 ? 128u : 64u;
   p.own = !decode && fixed % 64 == 0 && start % 64 == 0;
   const uint32_t s = fx_log2(W);
   p.skew = p.own ? 31u : s < 3 ? 3u : s;
@@ -198,11 +221,11 @@ ZX_HD bool fx_enc_store(uint64_t *words, uint64_t nwords, const uint64_t *lds, c
   if (gw >= nwords) return false;  // the reference drops writes past the end
   if constexpr (OWN) {
     const uint32_t W = fixed / 64;
-    words[gw] = lds[i + i / W];
+    fx_store_once(words + gw, lds[i + i / W]);
   } else {
     const uint64_t v = lds[fx_slot(i, p.skew)];
     if ((i == 0 && q.first_shared) || (i == q.nw - 1 && q.last_shared)) WordSink::merge(words + gw, v);
-    else words[gw] = v;
+    else fx_store_once(words + gw, v);
   }
   return true;
 }
@@ -214,7 +237,7 @@ ZX_HD void fx_dec_stage(uint64_t *lds, const uint64_t *words, uint64_t nwords, c
                         const FxGroup &q, uint32_t i)
 {
   const uint64_t gw = q.w0 + i;
-  const uint64_t v = gw < nwords ? words[gw] : 0ull;
+  const uint64_t v = gw < nwords ? fx_load_once(words + gw) : 0ull;
   const uint32_t s = fx_slot(i, p.skew);
   lds[s] = v;
   // the pad slot before the word holds a copy of it (StagedWords)
@@ -352,7 +375,7 @@ ZX_HD void op_store(uint64_t *words, uint64_t nwords, const uint64_t *lds, const
   const bool first = i == 0 && r.head, last = i == r.nw - 1 && r.tail;
   if (first) edges[0] = v;
   if (last) edges[1] = v;
-  if (!first && !last && r.w0 + i < nwords) words[r.w0 + i] = v;
+  if (!first && !last && r.w0 + i < nwords) fx_store_once(words + r.w0 + i, v);
 }
 
 // dense path, phase 2: word i of the group cleared where blocks may share it
