@@ -110,7 +110,8 @@ static uint64_t decompress_fx(const FieldDesc &f, const Cfg &c, const uint64_t *
   uint64_t maxword = 0;
   for (uint64_t g = 0; g < groups; g++) {
     const FxGroup q = fx_group(g, p.T, f.nblocks, start, fixed);
-    for (uint32_t i = 0; i < q.nw + 2; i++) fx_dec_stage(lds.data(), words, nwords, p, q, i);
+    // the kernel's staging phase, thread by thread
+    for (uint32_t t = 0; t < p.T; t++) fx_dec_stage_thread(lds.data(), words, nwords, p, q, t);
     for (uint32_t t = 0; t < q.nb; t++) {
       U v[Blk<D>::SIZE];
       const uint64_t loaded = p.flat ? fx_dec_block<S, D, REV, true>(lds.data(), p, q, t, fixed, v, c)

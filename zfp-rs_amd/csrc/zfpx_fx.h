@@ -232,16 +232,40 @@ ZX_HD bool fx_enc_store(uint64_t *words, uint64_t nwords, const uint64_t *lds, c
 
 // ------------------------------------------------------------------ decode
 
-// phase 1: stage word i (i < nw + 2: the reader looks up to two words ahead)
-ZX_HD void fx_dec_stage(uint64_t *lds, const uint64_t *words, uint64_t nwords, const FxPlan &p,
-                        const FxGroup &q, uint32_t i)
+// staged word i in its slot, and in the pad slot before it when it has one
+// (StagedWords)
+ZX_HD void fx_dec_put(uint64_t *lds, const FxPlan &p, uint32_t i, uint64_t v)
 {
-  const uint64_t gw = q.w0 + i;
-  const uint64_t v = gw < nwords ? fx_load_once(words + gw) : 0ull;
   const uint32_t s = fx_slot(i, p.skew);
   lds[s] = v;
-  // the pad slot before the word holds a copy of it (StagedWords)
   if (i != 0 && (i & ((1u << p.skew) - 1)) == 0) lds[s - 1] = v;
+}
+
+// Words a thread stages per batch: all of a batch's loads are issued before
+// the first is used, so their latencies overlap.
+constexpr uint32_t kFxBatch = 16;
+
+// phase 1 for thread t of T: stage words t, t + T, ... below nw + 2 (the
+// reader looks up to two words ahead), a batch of loads issued before the
+// batch is written
+ZX_HD void fx_dec_stage_thread(uint64_t *lds, const uint64_t *words, uint64_t nwords, const FxPlan &p,
+                               const FxGroup &q, uint32_t t)
+{
+  const uint32_t total = q.nw + 2;
+  for (uint32_t base = 0; base < total; base += kFxBatch * p.T) {
+    uint64_t v[kFxBatch];
+    ZX_UNROLL
+    for (uint32_t k = 0; k < kFxBatch; k++) {
+      const uint32_t i = base + t + k * p.T;
+      const uint64_t gw = q.w0 + i;
+      v[k] = i < total && gw < nwords ? fx_load_once(words + gw) : 0ull;
+    }
+    ZX_UNROLL
+    for (uint32_t k = 0; k < kFxBatch; k++) {
+      const uint32_t i = base + t + k * p.T;
+      if (i < total) fx_dec_put(lds, p, i, v[k]);
+    }
+  }
 }
 
 // phase 2: thread t decodes its block into v; returns one past the highest

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecod
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
   const FxGroup q = fx_group(blockIdx.x, p.T, a.field.nblocks, a.start, a.fixed);
-  for (uint32_t i = t; i < q.nw + 2; i += p.T) fx_dec_stage(lds, a.words, a.nwords, p, q, i);
+  fx_dec_stage_thread(lds, a.words, a.nwords, p, q, t);
   compress8_load<D>();
   __syncthreads();
   if (t >= q.nb) return;
