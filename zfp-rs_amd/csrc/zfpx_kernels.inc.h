@@ -219,6 +219,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kEncod
     fx_enc_block<S, D, REV, OWN>(lds, p, q, t, a.fixed, v, a.cfg);
   }
   __syncthreads();
+  if constexpr (OWN) {
+    // whole blocks of an even number of words from an even word: the words
+    // go out in 16-byte pairs (a pair never spans two blocks' slots)
+    const uint32_t W = a.fixed / 64;
+    if (W % 2 == 0 && q.w0 % 2 == 0 && q.w0 + q.nw <= a.nwords) {
+      for (uint32_t j = t; 2 * j < q.nw; j += p.T) {
+        const uint32_t i = 2 * j;
+        const uint32_t sl = i + i / W;
+        zx_u64x2 pr;
+        pr.x = lds[sl];
+        pr.y = lds[sl + 1];
+        __builtin_nontemporal_store(pr, (zx_u64x2 *)(a.words + q.w0 + i));
+      }
+      return;
+    }
+  }
   for (uint32_t i = t; i < q.nw; i += p.T)
     if (!fx_enc_store<OWN>(a.words, a.nwords, lds, p, q, i, a.fixed)) break;
 }
