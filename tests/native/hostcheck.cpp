@@ -123,6 +123,47 @@ static uint64_t decompress_fx(const FieldDesc &f, const Cfg &c, const uint64_t *
   return maxword;
 }
 
+// The variable-rate decode through LDS (k_decode_vx, then k_decode_dense for
+// the groups over the capacity), one workgroup and one thread at a time.
+template <class S, int D, bool REV>
+static uint64_t decompress_vx(const FieldDesc &f, const Cfg &c, const uint64_t *words,
+                              uint64_t nwords, uint64_t start, const uint32_t *lens, uint32_t T,
+                              uint32_t cap)
+{
+  typedef typename Ints<ST<S>::P>::U U;
+  std::vector<uint64_t> offs(f.nblocks + 1);
+  offs[0] = start;
+  for (uint64_t b = 0; b < f.nblocks; b++) offs[b + 1] = offs[b] + lens[b];
+  FxPlan p;
+  vx_plan(offs[f.nblocks] - start, f.nblocks, T, cap, p);
+  std::vector<uint64_t> lds(p.lds_words);
+  const uint64_t groups = (f.nblocks + p.T - 1) / p.T;
+  uint64_t maxword = 0;
+  for (uint64_t g = 0; g < groups; g++) {
+    const FxGroup q = vx_group(g, p.T, f.nblocks, offs.data());
+    for (auto &x : lds) x = 0x5555aaaa5555aaaaull;  // LDS starts undefined
+    const bool staged = q.nw <= p.cap;
+    if (staged)
+      for (uint32_t t = 0; t < p.T; t++) fx_dec_stage_thread(lds.data(), words, nwords, p, q, t);
+    for (uint32_t t = 0; t < q.nb; t++) {
+      U v[Blk<D>::SIZE];
+      const uint64_t b = q.b0 + t;
+      uint64_t loaded;
+      if (staged) {
+        loaded = vx_dec_block<S, D, REV>(lds.data(), p, q, offs[b], v, c);
+      } else {
+        WordSource r;
+        r.init(GlobalWords{words, nwords}, offs[b]);
+        decode_values<S, D, REV>(r, v, c);
+        loaded = loaded_words(r, offs[b], c.minbits);
+      }
+      if (b == f.nblocks - 1) maxword = loaded;
+      scatter<S, D>(f, block_pos<D>(f, b), v);
+    }
+  }
+  return maxword;
+}
+
 // The variable-rate LDS path: the length pass, the scan, then the write pass
 // one workgroup at a time, staged through LDS or, for a workgroup over the
 // capacity, block by block (the kernels' second launch; its blocks' bits are
@@ -282,6 +323,18 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
     case 1: return decompress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
     case 2: return decompress_fx<S, 2, REV>(f, c, w, nw, st, fx, p);
     default: return decompress_fx<S, 3, REV>(f, c, w, nw, st, fx, p);
+    }
+  }
+  if (T && !fx && l) {
+    // odd T: a capacity of the mean group's words, so that some groups are
+    // staged and some left to the dense pass
+    uint64_t bits = 0;
+    for (uint64_t b = 0; b < f.nblocks; b++) bits += l[b];
+    const uint32_t cap = T % 2 ? (uint32_t)((bits * T / (f.nblocks ? f.nblocks : 1) + 63) / 64) + 1 : 0u;
+    switch (dims) {
+    case 1: return decompress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);
+    case 2: return decompress_vx<S, 2, REV>(f, c, w, nw, st, l, T, cap);
+    default: return decompress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
     }
   }
   switch (dims) {

@@ -117,7 +117,7 @@ void run(const Case &k, bool special, uint32_t start)
     orc_field g{k.type, dz[0], dz[1], dz[2], 0, 0, 0, ref.data()};
     orc_decompress(&r, &k.cfg, &g, &err);
     hc_decompress(k.type, dims, rev, got.data(), n4, s4, &hc, hw.data(), cap, start, fixed,
-                  fixed ? nullptr : lens.data(), fixed ? T : 0);
+                  fixed ? nullptr : lens.data(), T);
     if (memcmp(ref.data(), got.data(), count * esz) != 0) {
       printf("FAIL values (type %d dims %d T %u fixed %u start %u special %d)\n", k.type, dims, T,
              fixed, start, special);

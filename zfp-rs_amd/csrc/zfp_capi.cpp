@@ -142,6 +142,7 @@ struct bitstream {
   size_t index_cap;
   uint64_t index_start;
   uint64_t index_key;
+  uint64_t index_bits;  // bits of the indexed blocks (sizes the decode's LDS staging; 0: unknown)
   // the batch call that last took this stream (duplicates within one call
   // go to the single-array calls)
   uint64_t batch_epoch;
@@ -336,6 +337,7 @@ void drop_index(bitstream *s)
   if (s->index) (void)hipFree(s->index);
   s->index = nullptr;
   s->index_n = s->index_cap = 0;
+  s->index_bits = 0;
 }
 
 }  // namespace
@@ -1014,6 +1016,7 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
     if (rb[1]) return false;  // a look-back gave up
     end_rel = rb[0] & ((1ull << 62) - 1);
     s->mean_bits = (uint32_t)std::min<uint64_t>((end_rel - a.start + p.nblocks - 1) / p.nblocks, 1u << 20);
+    if (keep_index) s->index_bits = end_rel - a.start;
     const uint64_t w1 = (64 * w0 + end_rel + 63) / 64;  // words through the flush
     const uint64_t live = std::min<uint64_t>(w1 - w0, nwords);
     if (!s->device && live) {
@@ -1082,7 +1085,10 @@ bool gpu_compress(bitstream *s, const zfp_field *f, const Plan &p, const zx::Cfg
   if (!s->device && live) {
     ZCHECK(hipMemcpyAsync(s->words + w0, dwords, live * 8, hipMemcpyDeviceToHost, st));
   }
-  if (!fixed && keep_index) keep_offset_index(s, lens, p, c, start, sc != nullptr, st);
+  if (!fixed && keep_index) {
+    keep_offset_index(s, lens, p, c, start, sc != nullptr, st);
+    s->index_bits = a.vbits;
+  }
   tm.mark(3);
   ZCHECK(hipStreamSynchronize(st));
   tm.collect();
@@ -1240,6 +1246,7 @@ bool gpu_decompress(bitstream *s, zfp_field *f, const Plan &p, const zx::Cfg &c,
       ZCHECK(hipMemcpyAsync(W.one, offs + p.nblocks, 8, hipMemcpyDeviceToHost, st));
       ZCHECK(hipMemcpyAsync((uint64_t *)W.one + 2, (uint64_t *)W.tiles + 1, 8, hipMemcpyDeviceToHost, st));
       a.offsets = offs;
+      a.vbits = s->index_bits;
     } else {
       walked = true;
     }
@@ -1821,6 +1828,7 @@ size_t batch_finish(const BatchSet &S, const BatchDev &D, zfp_stream *const *zs,
         s->index_n = it.plan.nblocks;
         s->index_start = it.start;
         s->index_key = cfg_key(cfg_of(z), it.plan);
+        s->index_bits = end - it.start;
       }
       if (!s->overflowed) {
         sizes[it.input] = s->word_pos * 8;
@@ -2883,6 +2891,9 @@ zfp_bool zfp_stream_index_import(bitstream *s, const uint32_t *lengths, size_t c
   s->index_n = count;
   s->index_start = start;
   s->index_key = 0;  // an imported index matches on start and block count
+  uint64_t bits = 0;
+  for (size_t i = 0; i < count; i++) bits += lengths[i];
+  s->index_bits = bits;
   return zfp_true;
 }
 

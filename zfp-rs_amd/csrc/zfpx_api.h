@@ -46,6 +46,8 @@ struct DecodeArgs {
   uint32_t slice;           // blocks per workgroup of the LDS path, or 0 for its default
   uint64_t first;           // variable placement: decode blocks [first, first + count) only,
   uint64_t count;           // or every block when count is 0
+  uint64_t vbits;           // variable placement through offsets: bits of all blocks, which
+                            // sizes the LDS staging (0: unknown, decode from HBM)
 };
 
 // One array of a batch (zfp_hip_compress_batch / zfp_hip_decompress_batch).

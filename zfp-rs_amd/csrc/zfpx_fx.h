@@ -130,9 +130,8 @@ ZX_HD uint32_t fx_slot(uint32_t i, uint32_t skew) { return i + (i >> skew); }
 // range is written block by block to HBM.  The LDS a launch reserves bounds
 // how many workgroups fit on a CU, so the capacity follows the stream's mean
 // block length: room for 1.5 mean groups, at most 64 KiB.  `T` (at most 256)
-// overrides the workgroup size and `cap` the capacity when nonzero.  (Decoding
-// variable-rate streams through LDS measured slower than reading HBM block by
-// block, so decode keeps the global-memory path.)
+// overrides the workgroup size and `cap` the capacity when nonzero.  Decodes
+// through an offset index stage their groups the same way (k_decode_vx).
 ZX_HD void vx_plan(uint64_t bits, uint64_t nblocks, uint32_t T, uint32_t cap, FxPlan &p)
 {
   const uint64_t mean = nblocks ? (bits + nblocks - 1) / nblocks : 0;
@@ -289,6 +288,21 @@ ZX_HD uint64_t fx_dec_block(const uint64_t *lds, const FxPlan &p, const FxGroup 
     decode_values<S, D, REV>(r, v, c);
     return q.w0 + loaded_words(r, pos, c.minbits);
   }
+}
+
+// Variable-rate decode from a staged group (vx_group's range, staged with
+// fx_dec_stage_thread): the block at stream bit `pos` of the words the
+// group's w0 counts from; returns one past the highest word its reader
+// would have loaded.
+template <class S, int D, bool REV>
+ZX_HD uint64_t vx_dec_block(const uint64_t *lds, const FxPlan &p, const FxGroup &q, uint64_t pos,
+                            typename Ints<ST<S>::P>::U (&v)[Blk<D>::SIZE], const Cfg &c)
+{
+  const uint64_t rel = pos - 64 * q.w0;
+  Source<StagedWords<false>> r;
+  r.init(StagedWords<false>{lds, p.skew}, rel);
+  decode_values<S, D, REV>(r, v, c);
+  return q.w0 + loaded_words(r, rel, c.minbits);
 }
 
 // ------------------------------------------------------------------ one pass

@@ -262,6 +262,59 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecod
   store_block<S, D>(a.field, b, a.vec, v);
 }
 
+// Variable-rate decode through LDS: a workgroup whose blocks span at most
+// p.cap words (by the offsets) stages them, its loads batched, and decodes
+// from LDS; the groups over the capacity are left to k_decode_dense.
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecodeWaves<S, REV>, 8))) void k_decode_vx(DecodeArgs a, FxPlan p)
+{
+  extern __shared__ uint64_t lds[];
+  const uint32_t t = threadIdx.x;
+  const FxGroup q = vx_group(blockIdx.x, p.T, a.field.nblocks, a.offsets);
+  if (q.nw > p.cap) return;
+  fx_dec_stage_thread(lds, a.words, a.nwords, p, q, t);
+  compress8_load<D>();
+  __syncthreads();
+  if (t >= q.nb) return;
+  const uint64_t b = q.b0 + t;
+  typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+  const uint64_t loaded = vx_dec_block<S, D, REV>(lds, p, q, a.offsets[b], v, a.cfg);
+  if (a.maxword && b == a.field.nblocks - 1) *a.maxword = loaded;
+  store_block<S, D>(a.field, b, a.vec, v);
+}
+
+// The groups k_decode_vx leaves: each block decoded from HBM at its offset,
+// as k_decode does; a workgroup tests blockDim.x groups at once (one per
+// thread) and then decodes the dense ones among them.
+template <class S, int D, bool REV>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_decode_dense(DecodeArgs a, FxPlan p, uint64_t ngroups)
+{
+  __shared__ uint32_t dense[kThreads];
+  __shared__ uint32_t ndense;
+  const uint32_t t = threadIdx.x;
+  if (t == 0) ndense = 0;
+  __syncthreads();
+  const uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x;
+  if (g0 + t < ngroups && vx_group(g0 + t, p.T, a.field.nblocks, a.offsets).nw > p.cap)
+    dense[atomicAdd(&ndense, 1u)] = t;
+  compress8_load<D>();
+  __syncthreads();
+  const uint32_t n = ndense;
+  for (uint32_t k = 0; k < n; k++) {
+    const FxGroup q = vx_group(g0 + dense[k], p.T, a.field.nblocks, a.offsets);
+    for (uint32_t j = t; j < q.nb; j += blockDim.x) {
+      const uint64_t b = q.b0 + j;
+      const uint64_t pos = a.offsets[b];
+      WordSource r;
+      r.init(GlobalWords{a.words, a.nwords}, pos);
+      typename Ints<ST<S>::P>::U v[Blk<D>::SIZE];
+      decode_values<S, D, REV>(r, v, a.cfg);
+      if (a.maxword && b == a.field.nblocks - 1) *a.maxword = loaded_words(r, pos, a.cfg.minbits);
+      store_block<S, D>(a.field, b, a.vec, v);
+    }
+  }
+}
+
 // Variable-rate write pass through LDS: a workgroup whose blocks touch at most
 // p.cap words encodes them into LDS at their scanned offsets and stores the
 // range with coalesced stores (atomic ORs for the two words it may share with
@@ -526,6 +579,15 @@ static hipError_t launch_decode_t(const DecodeArgs &a, hipStream_t s)
       const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
       if (p.flat) hipLaunchKernelGGL((k_decode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
       else hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+      return hipGetLastError();
+    }
+    if (a.offsets && a.vbits && !a.count) {
+      // one wave per workgroup, as the fixed-rate decode
+      vx_plan(a.vbits, a.field.nblocks, a.slice ? a.slice : 64u, 0, p);
+      const uint64_t groups = (a.field.nblocks + p.T - 1) / p.T;
+      hipLaunchKernelGGL((k_decode_vx<S, D, REV>), dim3(groups), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+      hipLaunchKernelGGL((k_decode_dense<S, D, REV>), dim3((groups + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                         a, p, groups);
       return hipGetLastError();
     }
   }
