@@ -927,6 +927,11 @@ ZX_HD uint64_t compress_ones(uint64_t v, uint32_t nbits)
   return out;
 }
 
+// Dimensionalities whose decoders parse planes through the compress8 table
+// (64-bit windows and 32-bit windows for 3-D blocks, 32-bit windows for 2-D
+// blocks); a kernel that decodes them fills the workgroup's copy first.
+template <int D> constexpr bool kDecodeTable = D == 2 || D == 3;
+
 // compress_ones of a value below 2^32: the four byte lookups only.
 ZX_HD uint32_t compress_ones32(uint32_t v)
 {
@@ -1342,10 +1347,10 @@ ZX_HD void plane_steps(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &o
 }
 
 // plane_tests for a wave whose lanes all have at most 16 coefficients left
-// (n >= SIZE - 16): each positive test is followed by at most two bits per
-// remaining coefficient, the last one implied, so the tests after the first
-// take at most 30 bits and one 32-bit window ends every lane's plane; the
-// same parse in 32-bit arithmetic.
+// (n >= SIZE - 16; every plane of a 2-D block): each positive test is
+// followed by at most two bits per remaining coefficient, the last one
+// implied, so the tests after the first take at most 30 bits and one 32-bit
+// window ends every lane's plane; the same parse in 32-bit arithmetic.
 template <int SIZE, class Src>
 ZX_HD void plane_tests_narrow(Src &r, uint32_t &n, uint32_t &bits, uint64_t &x, bool &one, bool go,
                               uint64_t ahead)
@@ -1515,7 +1520,7 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
       if (act) {
         m = umin(n, bits);
         bits -= m;
-        if constexpr (SIZE == 64) x = r.read_peek(m, ahead);
+        if constexpr (SIZE >= 16) x = r.read_peek(m, ahead);
         else x = r.read(m);
         tested = bits != 0 && n < (uint32_t)SIZE;
       }
@@ -1525,6 +1530,10 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
           plane_tests_narrow<SIZE>(r, n, bits, x, one, tested, ahead);
         else
           plane_tests<SIZE>(r, n, bits, x, one, tested, ahead);
+      } else if constexpr (SIZE == 16) {
+        // every plane of a 16-coefficient block fits one 32-bit window
+        static_assert(kDecodeTable<2>, "2-D decoders read the compress8 table");
+        plane_tests_narrow<SIZE>(r, n, bits, x, one, tested, ahead);
       } else {
         plane_tests<SIZE>(r, n, bits, x, one, tested, 0);
       }

@@ -58,21 +58,22 @@ __device__ __forceinline__ void expand8_fill()
 }
 
 // Fills the workgroup's compress8 table (zfpx_block.h), which the decoders of
-// 3-D blocks read; every kernel that decodes them calls it first, with all of
-// its threads, before any thread leaves.  Other dimensionalities skip it.
+// 2-D and 3-D blocks read (kDecodeTable); every kernel that decodes them
+// calls it first, with all of its threads, before any thread leaves.  Other
+// dimensionalities skip it.
 template <int D>
 __device__ __forceinline__ void compress8_load()
 {
-  if constexpr (D == 3)
+  if constexpr (kDecodeTable<D>)
     for (uint32_t i = threadIdx.x; i < 64; i += blockDim.x)
       ((uint32_t *)zx_compress8_lds)[i] = ((const uint32_t *)kCompress8Dev.v)[i];
 }
 
-// compress8_load and the barrier after it (3-D only)
+// compress8_load and the barrier after it (2-D and 3-D only)
 template <int D>
 __device__ __forceinline__ void compress8_fill()
 {
-  if constexpr (D == 3) {
+  if constexpr (kDecodeTable<D>) {
     compress8_load<D>();
     __syncthreads();
   }
