@@ -1222,40 +1222,51 @@ ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool r
 // reads them: packed 16-bit halves of each 16-plane chunk, and the bit length
 // of their OR (planes above it are empty).
 // Returns the OR of the coefficients.
-template <int P, int SIZE>
+// DNB ("deferred negabinary"): u holds the coefficients before the
+// negabinary XOR (q + NB), which encode_halves applies to the transposed
+// planes instead (the XOR with NB inverts every odd plane), and top is P:
+// coding the leading empty planes one by one gives the same bits as skipping
+// them (each is a lone negative test while no coefficient is significant),
+// and a float block's planes start at or near the top anyway.
+template <int P, int SIZE, bool DNB = false>
 ZX_HD typename Ints<P>::U prepare_ints(const typename Ints<P>::U (&u)[SIZE],
                                        Halves<SIZE> (&hv)[P / kEncodeChunk], uint32_t &top)
 {
   typedef typename Ints<P>::U U;
   U acc = 0;
-  ZX_UNROLL
-  for (int i = 0; i < SIZE; i++) acc |= u[i];
-  top = P == 32 ? 32u - clz32((uint32_t)acc) : 64u - clz64((uint64_t)acc);
+  if constexpr (DNB) {
+    top = P;
+  } else {
+    ZX_UNROLL
+    for (int i = 0; i < SIZE; i++) acc |= u[i];
+    top = P == 32 ? 32u - clz32((uint32_t)acc) : 64u - clz64((uint64_t)acc);
+  }
   ZX_UNROLL
   for (int c = 0; c < P / kEncodeChunk; c++) chunk_halves<P, SIZE>(u, c, hv[c]);
   return acc;
 }
 
-template <int P, int SIZE, class Sink>
+template <int P, int SIZE, bool DNB = false, class Sink>
 ZX_HD uint32_t encode_halves(Sink &w, uint32_t maxbits, uint32_t maxprec,
                              const Halves<SIZE> (&hv)[P / kEncodeChunk], uint32_t top);
 
-template <int P, int SIZE, class Sink>
+template <int P, int SIZE, bool DNB = false, class Sink>
 ZX_HD uint32_t encode_ints(Sink &w, uint32_t maxbits, uint32_t maxprec,
                            const typename Ints<P>::U (&u)[SIZE])
 {
   if constexpr (SIZE == 256) {
+    static_assert(!DNB, "4-D blocks take the XOR before their planes");
     return encode_ints256<P>(w, maxbits, maxprec, u);
   } else {
     Halves<SIZE> hv[P / kEncodeChunk];
     uint32_t top;
-    prepare_ints<P, SIZE>(u, hv, top);
-    return encode_halves<P, SIZE>(w, maxbits, maxprec, hv, top);
+    prepare_ints<P, SIZE, DNB>(u, hv, top);
+    return encode_halves<P, SIZE, DNB>(w, maxbits, maxprec, hv, top);
   }
 }
 
 // encode_ints and encode_planes (bitplane.rs:539-610) from prepared chunks.
-template <int P, int SIZE, class Sink>
+template <int P, int SIZE, bool DNB, class Sink>
 ZX_HD uint32_t encode_halves(Sink &w, uint32_t maxbits, uint32_t maxprec,
                              const Halves<SIZE> (&hv)[P / kEncodeChunk], uint32_t top)
 {
@@ -1273,30 +1284,40 @@ ZX_HD uint32_t encode_halves(Sink &w, uint32_t maxbits, uint32_t maxprec,
   uint32_t n = 0;
   constexpr int CH = kEncodeChunk;
   static_assert(CH == 16, "the encoder's chunks are 16 planes");
+  // plane k's code, cut to the budget
+  auto code_plane = [&](PW plane, uint32_t k) {
+    if (bits != 0 && k < kstart && k >= kmin) {
+      uint64_t clo = 0, chi = 0;
+      uint32_t nn;
+      uint32_t len;
+      if constexpr (sink_counts<Sink>::value) len = plane_len<SIZE>((uint64_t)plane, n, nn);
+      else len = plane_code<SIZE>((uint64_t)plane, n, clo, chi, nn);
+      const uint32_t take = umin(len, bits);
+      if constexpr (Sink::kSpillPastBudget)
+        w.put128(clo, chi, take);
+      else
+        w.put128(clo & lowmask(take), take > 64 ? chi & lowmask(take - 64) : 0ull, take);
+      bits -= take;
+      n = nn;
+    }
+  };
+  // the XOR with NB deferred to the planes (DNB) inverts every odd plane
+  auto fix_planes = [&](PW (&ch)[CH]) {
+    if constexpr (DNB) {
+      static_assert(SIZE == 64, "planes of whole words");
+      ZX_UNROLL
+      for (int i = 1; i < CH; i += 2) ch[i] = ~ch[i];
+    }
+  };
   ZX_UNROLL
   for (int c = P / CH - 1; c >= 0; c--) {
     const bool need = bits != 0 && kstart > (uint32_t)(CH * c) && kmin <= (uint32_t)(CH * c + CH - 1);
     if (!wave_any(need)) continue;
     PW ch[CH];
     chunk_planes<SIZE>(hv[c], ch);
+    fix_planes(ch);
     ZX_UNROLL
-    for (int i = CH - 1; i >= 0; i--) {
-      const uint32_t k = (uint32_t)(CH * c + i);
-      if (bits != 0 && k < kstart && k >= kmin) {
-        uint64_t clo = 0, chi = 0;
-        uint32_t nn;
-        uint32_t len;
-        if constexpr (sink_counts<Sink>::value) len = plane_len<SIZE>((uint64_t)ch[i], n, nn);
-        else len = plane_code<SIZE>((uint64_t)ch[i], n, clo, chi, nn);
-        const uint32_t take = umin(len, bits);
-        if constexpr (Sink::kSpillPastBudget)
-          w.put128(clo, chi, take);
-        else
-          w.put128(clo & lowmask(take), take > 64 ? chi & lowmask(take - 64) : 0ull, take);
-        bits -= take;
-        n = nn;
-      }
-    }
+    for (int i = CH - 1; i >= 0; i--) code_plane(ch[i], (uint32_t)(CH * c + i));
   }
   return budget - bits;
   }
@@ -1569,7 +1590,7 @@ ZX_HD uint32_t decode_ints(Src &r, uint32_t maxbits, uint32_t maxprec, bool roun
 
 // ---------------------------------------------------------------- integer blocks
 
-template <int P, int D, class Sink>
+template <int P, int D, bool DNB = false, class Sink>
 ZX_HD uint32_t encode_int_block(Sink &w, typename Ints<P>::I (&q)[Blk<D>::SIZE], uint32_t minbits,
                                 uint32_t maxbits, uint32_t maxprec, int rounding)
 {
@@ -1585,8 +1606,9 @@ ZX_HD uint32_t encode_int_block(Sink &w, typename Ints<P>::I (&q)[Blk<D>::SIZE],
   }
   U u[SIZE];
   ZX_UNROLL
-  for (int j = 0; j < SIZE; j++) u[j] = ((U)q[perm(D, j)] + Ints<P>::NB) ^ Ints<P>::NB;
-  uint32_t bits = encode_ints<P, SIZE>(w, maxbits, maxprec, u);
+  for (int j = 0; j < SIZE; j++)
+    u[j] = DNB ? (U)q[perm(D, j)] + Ints<P>::NB : ((U)q[perm(D, j)] + Ints<P>::NB) ^ Ints<P>::NB;
+  uint32_t bits = encode_ints<P, SIZE, DNB>(w, maxbits, maxprec, u);
   if (bits < minbits) {
     w.zeros(minbits - bits);
     bits = minbits;
@@ -1851,8 +1873,9 @@ ZX_HD uint32_t encode_lossy(Sink &w, const typename Ints<ST<S>::P>::U (&b)[Blk<D
       const uint32_t hb = ST<S>::EBITS + 1;
       w.put(2ull * e + 1, hb);
       cast_block<S, SIZE>(b, emax, special, q);
-      return hb + encode_int_block<P, D>(w, q, sat_sub(c.minbits, hb), sat_sub(c.maxbits, hb), prec,
-                                         c.rounding);
+      // 3-D float blocks take the negabinary XOR on their planes (DNB)
+      return hb + encode_int_block<P, D, D == 3>(w, q, sat_sub(c.minbits, hb), sat_sub(c.maxbits, hb),
+                                                 prec, c.rounding);
     }
     w.put(0, 1);
     if (c.minbits > 1) {
