@@ -691,15 +691,19 @@ struct StagedWords {
     lo = q[0];
     hi = q[1];
   }
-  // FLAT: words i, i + 1 and i + 2 for the word i holding bit p, from one
-  // base (the third may be a word past the block's pad copy: a neighbour's
-  // word or a spare slot, whose bits the block's budget never lets a reader use)
+  // words i, i + 1 and i + 2 for the word i holding bit p (the third may lie
+  // past the words staged for the reader's block: a neighbour's word or a
+  // spare slot, whose bits end no plane the reader parses).  Not FLAT: when
+  // i + 1 is a multiple of 2^skew, the slot after word i holds word i + 1's
+  // pad copy and word i + 2 is one slot further.
   ZX_HD void triple_at(uint32_t p, uint64_t &w0, uint64_t &w1, uint64_t &w2) const
   {
-    const uint64_t *q = w + (p >> 6);
+    const uint32_t i = p >> 6;
+    const uint64_t *q = FLAT ? w + i : w + (i + (i >> skew));
     w0 = q[0];
     w1 = q[1];
-    w2 = q[2];
+    if constexpr (FLAT) w2 = q[2];
+    else w2 = q[((i + 1) & ((1u << skew) - 1)) == 0 ? 3 : 2];
   }
 };
 
@@ -789,27 +793,21 @@ struct Source<StagedWords<FLAT>> {
     consume(n);
     return v & lowmask(n);
   }
-  // read(n) (n <= 64), and the 64 bits that follow in `ahead`: over a FLAT
-  // block both come from one load of the three words under them, so a
-  // plane's verbatim bits and its first test window cost one LDS round trip
+  // read(n) (n <= 64), and the 64 bits that follow in `ahead`: both come
+  // from one load of the three words under them, so a plane's verbatim bits
+  // and its first test window cost one LDS round trip
   ZX_HD uint64_t read_peek(uint32_t n, uint64_t &ahead)
   {
-    if constexpr (FLAT) {
-      const uint32_t s = p & 63;
-      uint64_t w0, w1, w2;
-      mem.triple_at(p, w0, w1, w2);
-      const uint64_t v = (w0 >> s) | ((w1 << 1) << (63 - s));
-      const uint32_t t = s + n;  // 0 .. 127
-      const uint32_t u = t & 63;
-      const uint64_t a = t < 64 ? w0 : w1, b = t < 64 ? w1 : w2;
-      ahead = (a >> u) | ((b << 1) << (63 - u));
-      p += n;
-      return v & lowmask(n);
-    } else {
-      const uint64_t v = read(n);
-      ahead = peek();
-      return v;
-    }
+    const uint32_t s = p & 63;
+    uint64_t w0, w1, w2;
+    mem.triple_at(p, w0, w1, w2);
+    const uint64_t v = (w0 >> s) | ((w1 << 1) << (63 - s));
+    const uint32_t t = s + n;  // 0 .. 127
+    const uint32_t u = t & 63;
+    const uint64_t a = t < 64 ? w0 : w1, b = t < 64 ? w1 : w2;
+    ahead = (a >> u) | ((b << 1) << (63 - u));
+    p += n;
+    return v & lowmask(n);
   }
   ZX_HD uint64_t pos() const { return p; }
   ZX_HD uint64_t words_loaded() const { return (p + 63) >> 6; }

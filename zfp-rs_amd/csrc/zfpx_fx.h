@@ -140,11 +140,12 @@ ZX_HD void vx_plan(uint64_t bits, uint64_t nblocks, uint32_t T, uint32_t cap, Fx
   p.skew = 3;
   p.own = false;
   p.flat = false;
-  // words 0 .. cap + 1 in slots
-  const uint64_t most = (kFxMaxLdsBytes / 8 - 1) * 8 / 9 - 2;
+  // words 0 .. cap + 1 staged, and slots up to word cap + 2 (a reader near
+  // the end loads three words: Source<StagedWords>::read_peek)
+  const uint64_t most = (kFxMaxLdsBytes / 8 - 1) * 8 / 9 - 3;
   const uint64_t want = cap ? cap : (mean * p.T * 3 / 2 + 63) / 64 + 16;
   p.cap = (uint32_t)(want < most ? want : most);
-  p.lds_words = fx_slot(p.cap + 1, p.skew) + 1;
+  p.lds_words = fx_slot(p.cap + 2, p.skew) + 1;
 }
 
 ZX_HD FxGroup vx_group(uint64_t g, uint32_t T, uint64_t nblocks, const uint64_t *offsets)
