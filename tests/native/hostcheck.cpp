@@ -100,16 +100,31 @@ static uint64_t compress_fx(const FieldDesc &f, const Cfg &c, uint64_t *words, u
   return start + f.nblocks * fixed;
 }
 
+// LDS starts undefined: each group's copy is filled with different words
+// first, so that a read of a slot the group did not stage shows as a
+// mismatch with the oracle
+static void lds_garbage(std::vector<uint64_t> &lds, uint64_t g)
+{
+  uint64_t z = 0x9e3779b97f4a7c15ull * (g + 1);
+  for (auto &x : lds) {
+    z ^= z << 13;
+    z ^= z >> 7;
+    z ^= z << 17;
+    x = z;
+  }
+}
+
 template <class S, int D, bool REV>
 static uint64_t decompress_fx(const FieldDesc &f, const Cfg &c, const uint64_t *words,
                               uint64_t nwords, uint64_t start, uint32_t fixed, const FxPlan &p)
 {
   typedef typename Ints<ST<S>::P>::U U;
-  std::vector<uint64_t> lds(p.lds_words, 0x5555aaaa5555aaaaull);
+  std::vector<uint64_t> lds(p.lds_words);
   const uint64_t groups = (f.nblocks + p.T - 1) / p.T;
   uint64_t maxword = 0;
   for (uint64_t g = 0; g < groups; g++) {
     const FxGroup q = fx_group(g, p.T, f.nblocks, start, fixed);
+    lds_garbage(lds, g);
     // the kernel's staging phase, thread by thread
     for (uint32_t t = 0; t < p.T; t++) fx_dec_stage_thread(lds.data(), words, nwords, p, q, t);
     for (uint32_t t = 0; t < q.nb; t++) {
@@ -141,7 +156,7 @@ static uint64_t decompress_vx(const FieldDesc &f, const Cfg &c, const uint64_t *
   uint64_t maxword = 0;
   for (uint64_t g = 0; g < groups; g++) {
     const FxGroup q = vx_group(g, p.T, f.nblocks, offs.data());
-    for (auto &x : lds) x = 0x5555aaaa5555aaaaull;  // LDS starts undefined
+    lds_garbage(lds, g);
     const bool staged = q.nw <= p.cap;
     if (staged)
       for (uint32_t t = 0; t < p.T; t++) fx_dec_stage_thread(lds.data(), words, nwords, p, q, t);
