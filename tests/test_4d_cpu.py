@@ -79,3 +79,52 @@ def test_4d_device_codec_matches_oracle(ztype):
             got, _ = H.decompress(shape, arr.dtype, ztype, cfg, words, 0, fixed,
                                   None if fixed else lens, rev)
             assert np.array_equal(got.view(u), ref.view(u)), ctx
+
+
+def sparse(shape, dt, seed):
+    """Mostly zero, with a few spikes: long zero scans across plane words."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros(shape, dtype=dt)
+    flat = a.reshape(-1)
+    idx = rng.choice(flat.size, size=max(1, flat.size // 40), replace=False)
+    if np.dtype(dt).kind == "f":
+        flat[idx] = rng.standard_normal(idx.size) * 10.0 ** rng.integers(-3, 4, idx.size)
+    else:
+        info = np.iinfo(dt)
+        flat[idx] = rng.integers(info.min // 4, info.max // 4, idx.size, dtype=dt)
+    return a
+
+
+@pytest.mark.parametrize("ztype", TYPES)
+def test_4d_coder_many_blocks_and_budgets(ztype):
+    """24 blocks (2 x 2 x 2 x 3, partial in x) of smooth, random-walk and
+    sparse data under budgets that end planes at many places (rates that are
+    not whole planes, low precisions, ROUND_LAST under a binding rate)."""
+    shape = (8, 8, 8, 11)
+    dt = O.DTYPES[ztype]
+    cfgs = [("rate%g" % r, O.fixed_rate(r, ztype, 4)) for r in (0.5, 2.5, 5, 8, 12.25, 20, 31)]
+    cfgs += [("prec%d" % p, O.fixed_precision(p)) for p in (3, 17, 40)]
+    cfgs += [("rev", O.reversible())]
+    b = O.fixed_rate(6, ztype, 4)
+    cfgs.append(("rate6_last", O.config(b.minbits, b.maxbits, b.maxprec, b.minexp, O.ROUND_LAST, True)))
+    if ztype in (O.FLOAT, O.DOUBLE):
+        cfgs.append(("acc1e-3", O.fixed_accuracy(1e-3)))
+    for kind in ("smooth", "walk", "sparse"):
+        arr = sparse(shape, dt, 5 + ztype) if kind == "sparse" else data(ztype, shape, kind, seed=3 * ztype + 1)
+        u = np.uint32 if arr.dtype.itemsize == 4 else np.uint64
+        for name, cfg in cfgs:
+            ctx = f"{name} {kind}"
+            cap = 24 * 16658 // 64 + 64
+            ow, offs = O.compress_blocks(arr, cfg, cap)
+            nwords = (int(offs[-1]) + 63) // 64
+            ref = O.decompress_blocks(ow, shape, arr.dtype, cfg)
+            fixed = cfg.maxbits if (cfg.minbits == cfg.maxbits and
+                                    O.lib().orc_block_bits(cfg, ztype, 4) == cfg.maxbits) else 0
+            rev = cfg.minexp < -1074
+            words = np.zeros(nwords + 4, dtype=np.uint64)
+            end, lens = H.compress(arr, ztype, cfg, words, 0, fixed, rev)
+            assert end == offs[-1], ctx
+            assert np.array_equal(words[:nwords], ow[:nwords]) and not words[nwords:].any(), ctx
+            got, _ = H.decompress(shape, arr.dtype, ztype, cfg, words, 0, fixed,
+                                  None if fixed else lens, rev)
+            assert np.array_equal(got.view(u), ref.view(u)), ctx

@@ -1066,22 +1066,73 @@ ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, 
   return n + glen;
 }
 
-// encode_ints + encode_planes (bitplane.rs:509-610): each plane's code is
-// built in registers and appended once, cut to the budget.  Planes are
-// transposed 16 at a time (a chunk), and a chunk no lane of the wave codes
-// is skipped.  Returns bits written.
-// 4-D blocks: the same coder over planes of 256 coefficients (bitplane.rs:
-// 161-218 keeps such a plane in four words), written as C's encode_ints and
-// decode_ints are, one plane and one group test at a time.  A 4-D block's
-// values do not fit in registers, so these loops run over arrays the compiler
-// keeps in scratch memory; 4-D fields are coded for completeness, not speed.
+// 4-D blocks: planes of 256 coefficients (bitplane.rs:161-218 keeps such a
+// plane in four words), coded as C's encode_ints and decode_ints are, one
+// group test at a time.  The planes come from the 64-coefficient chunk
+// transposes applied to each quarter of the block, and decoded planes go back
+// through the 64-coefficient chunk folds, so every coefficient access has a
+// fixed index and the block stays in registers.
+template <class T> using Quarter = T[64];
+template <class T> ZX_HD Quarter<T> &quarter(T (&a)[256], int g)
+{
+  return *reinterpret_cast<Quarter<T> *>(a + 64 * g);
+}
+template <class T> ZX_HD const Quarter<T> &quarter(const T (&a)[256], int g)
+{
+  return *reinterpret_cast<const Quarter<T> *>(a + 64 * g);
+}
+
+// Position of the first one-bit at or after n in a 256-bit plane, or 256.
 ZX_HD uint32_t next_one256(const uint64_t (&x)[4], uint32_t n)
 {
-  for (uint32_t c = n >> 6; c < 4; c++) {
-    const uint64_t v = c == (n >> 6) ? x[c] >> (n & 63) << (n & 63) : x[c];
-    if (v) return 64 * c + ctz64(v);
+  uint32_t q = 256;
+  ZX_UNROLL
+  for (int c = 3; c >= 0; c--) {
+    const uint32_t lo = 64u * c;
+    const uint64_t keep = n <= lo ? ~0ull : n >= lo + 64 ? 0ull : ~0ull << (n - lo);
+    const uint64_t v = x[c] & keep;
+    q = v ? lo + ctz64(v) : q;
   }
-  return 256;
+  return q;
+}
+
+// One 256-coefficient plane x, n coefficients already significant, at most
+// `bits` written (encode_planes, bitplane.rs:561-610).
+template <class Sink>
+ZX_HD void code_plane256(Sink &w, uint32_t &bits, uint32_t &n, const uint64_t (&x)[4])
+{
+  // the n coefficients already significant, verbatim
+  const uint32_t m = umin(n, bits);
+  bits -= m;
+  ZX_UNROLL
+  for (int h = 0; h < 2; h++) {
+    const uint32_t base = 128u * h;
+    if (m > base) {
+      const uint32_t t = umin(m - base, 128u);
+      const uint64_t lo = t >= 64 ? x[2 * h] : x[2 * h] & lowmask(t);
+      const uint64_t hi = t > 64 ? x[2 * h + 1] & lowmask(t - 64) : 0ull;
+      w.put128(lo, hi, t);
+    }
+  }
+  // group tests: 1, the zeros before the next one-bit, and its 1 (implied
+  // for the last coefficient); 0 when no one-bit is left
+  while (bits != 0 && n < 256) {
+    const uint32_t q = next_one256(x, n);
+    if (q == 256) {
+      w.zeros(1);
+      bits -= 1;
+      break;
+    }
+    const uint32_t zeros = q - n;
+    const bool last = q == 255;
+    const uint32_t len = zeros + 2 - (last ? 1u : 0u);
+    const uint32_t take = umin(len, bits);
+    w.put128(1ull, 0ull, 1);
+    w.zeros(umin(zeros, take - 1));
+    if (take == len && !last) w.put128(1ull, 0ull, 1);
+    bits -= take;
+    n = q + 1;
+  }
 }
 
 template <int P, class Sink>
@@ -1092,6 +1143,7 @@ ZX_HD uint32_t encode_ints256(Sink &w, uint32_t maxbits, uint32_t maxprec,
   const uint32_t budget = with_maxbits(maxbits, maxprec, 256) ? maxbits : 0xffffffffu;
   const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
   U acc = 0;
+  ZX_UNROLL
   for (int i = 0; i < 256; i++) acc |= u[i];
   const uint32_t top = P == 32 ? 32u - clz32((uint32_t)acc) : 64u - clz64((uint64_t)acc);
   uint32_t bits = budget;
@@ -1099,41 +1151,28 @@ ZX_HD uint32_t encode_ints256(Sink &w, uint32_t maxbits, uint32_t maxprec,
   if (empty > bits) empty = bits;
   w.zeros(empty);
   bits -= empty;
-  uint32_t k = (uint32_t)P - empty, n = 0;
-  while (bits != 0 && k > kmin) {
-    k--;
-    uint64_t x[4];
-    for (int c = 0; c < 4; c++) {
-      uint64_t v = 0;
-      for (int i = 0; i < 64; i++) v |= (uint64_t)((u[64 * c + i] >> k) & 1u) << i;
-      x[c] = v;
+  // planes kstart - 1 down to kmin are coded while budget remains
+  const uint32_t kstart = (uint32_t)P - empty;
+  uint32_t n = 0;
+  constexpr int CH = kEncodeChunk;
+  ZX_UNROLL
+  for (int c = P / CH - 1; c >= 0; c--) {
+    const bool need = bits != 0 && kstart > (uint32_t)(CH * c) && kmin <= (uint32_t)(CH * c + CH - 1);
+    if (!wave_any(need)) continue;
+    uint64_t xs[CH][4];
+    ZX_UNROLL
+    for (int g = 0; g < 4; g++) {
+      Halves<64> hv;
+      chunk_halves<P, 64>(quarter(u, g), c, hv);
+      uint64_t pl[CH];
+      chunk_planes<64>(hv, pl);
+      ZX_UNROLL
+      for (int i = 0; i < CH; i++) xs[i][g] = pl[i];
     }
-    // the n coefficients already significant, verbatim
-    uint32_t m = umin(n, bits);
-    bits -= m;
-    for (int c = 0; m != 0; c++) {
-      const uint32_t t = umin(m, 64);
-      w.put128(x[c] & lowmask(t), 0ull, t);
-      m -= t;
-    }
-    // group tests: 1, the zeros before the next one-bit, and its 1 (implied
-    // for the last coefficient); 0 when no one-bit is left
-    while (bits != 0 && n < 256) {
-      const uint32_t q = next_one256(x, n);
-      if (q == 256) {
-        w.zeros(1);
-        bits -= 1;
-        break;
-      }
-      const uint32_t zeros = q - n;
-      const bool last = q == 255;
-      const uint32_t len = zeros + 2 - (last ? 1u : 0u);
-      const uint32_t take = umin(len, bits);
-      w.put128(1ull, 0ull, 1);
-      w.zeros(umin(zeros, take - 1));
-      if (take == len && !last) w.put128(1ull, 0ull, 1);
-      bits -= take;
-      n = q + 1;
+    ZX_UNROLL
+    for (int i = CH - 1; i >= 0; i--) {
+      const uint32_t k = (uint32_t)(CH * c + i);
+      if (bits != 0 && k < kstart && k >= kmin) code_plane256(w, bits, n, xs[i]);
     }
   }
   return budget - bits;
@@ -1147,57 +1186,69 @@ ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool r
   const bool constrained = with_maxbits(maxbits, maxprec, 256);
   const uint32_t budget = constrained ? maxbits : 0xffffffffu;
   const uint32_t kmin = (uint32_t)P > maxprec ? (uint32_t)P - maxprec : 0u;
+  ZX_UNROLL
   for (int i = 0; i < 256; i++) u[i] = 0;
   uint32_t bits = budget, n = 0, m = 0, k = P;
   bool act = true;
-  for (int plane = P - 1; plane >= 0; plane--) {
-    if (bits == 0) {
-      act = false;
-      break;
-    }
-    k -= 1;
-    m = 0;
-    if (k < kmin) {
-      act = false;
-      break;
-    }
-    m = umin(n, bits);
-    bits -= m;
-    uint64_t x[4] = {0, 0, 0, 0};
-    for (uint32_t c = 0, left = m; left != 0; c++) {
-      const uint32_t t = umin(left, 64);
-      x[c] = r.read(t);
-      left -= t;
-    }
-    const bool tested = bits != 0 && n < 256;
-    bool one = false;
-    while (bits != 0 && n < 256) {
-      bits -= 1;
-      one = r.read(1) != 0;
-      if (!one) break;
-      // the zeros before the next one-bit (read up to 64 at a time), up to
-      // the last coefficient, whose one-bit is implied, or the budget
-      for (uint32_t left = umin(255 - n, bits); left != 0;) {
-        const uint32_t span = umin(left, 64);
-        const uint32_t z = ctz64(r.peek());
-        if (z < span) {
-          r.consume(z + 1);
-          bits -= z + 1;
-          n += z;
-          break;
+  constexpr int CH = kDecodeChunk;
+  ZX_UNROLL
+  for (int c = P / CH - 1; c >= 0; c--) {
+    if (!wave_any(act)) break;
+    uint64_t pl[4][CH];
+    ZX_UNROLL
+    for (int i = CH - 1; i >= 0; i--) {
+      // as decode_ints: a plane is read while budget remains and k > kmin
+      const bool live = act && bits != 0;
+      k -= live ? 1u : 0u;  // == CH*c + i when live
+      m = live ? 0u : m;
+      act = live && k >= kmin;
+      uint64_t x[4] = {0, 0, 0, 0};
+      bool tested = false, one = false;
+      if (act) {
+        m = umin(n, bits);
+        bits -= m;
+        ZX_UNROLL
+        for (int q = 0; q < 4; q++) {
+          const uint32_t lo = 64u * q;
+          if (m > lo) x[q] = r.read(umin(m - lo, 64u));
         }
-        r.consume(span);
-        bits -= span;
-        n += span;
-        left -= span;
+        tested = bits != 0 && n < 256;
+        while (bits != 0 && n < 256) {
+          bits -= 1;
+          one = r.read(1) != 0;
+          if (!one) break;
+          // the zeros before the next one-bit (read up to 64 at a time), up
+          // to the last coefficient, whose one-bit is implied, or the budget
+          for (uint32_t left = umin(255 - n, bits); left != 0;) {
+            const uint32_t span = umin(left, 64);
+            const uint32_t z = ctz64(r.peek());
+            if (z < span) {
+              r.consume(z + 1);
+              bits -= z + 1;
+              n += z;
+              break;
+            }
+            r.consume(span);
+            bits -= span;
+            n += span;
+            left -= span;
+          }
+          ZX_UNROLL
+          for (int q = 0; q < 4; q++) x[q] |= (n >> 6) == (uint32_t)q ? 1ull << (n & 63) : 0ull;
+          n++;
+        }
       }
-      x[n >> 6] |= 1ull << (n & 63);
-      n++;
+      if (tested) m = one ? n : 256u;
+      ZX_UNROLL
+      for (int q = 0; q < 4; q++) pl[q][i] = x[q];
     }
-    if (tested) m = one ? n : 256u;
-    // fold the plane's one-bits in: only those coefficients change
-    for (int c = 0; c < 4; c++)
-      for (uint64_t b = x[c]; b != 0; b &= b - 1) u[64 * c + ctz64(b)] |= (U)1 << k;
+    ZX_UNROLL
+    for (int g = 0; g < 4; g++) {
+      uint64_t any = 0;
+      ZX_UNROLL
+      for (int i = 0; i < CH; i++) any |= pl[g][i];
+      if (wave_any(any != 0)) chunk_to_coeffs<P, 64, CH>(pl[g], c, quarter(u, g));
+    }
   }
   if (act && bits != 0) {  // ran through plane 0 with budget left (kmin == 0)
     m = 0;
@@ -1209,6 +1260,7 @@ ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool r
     if (rp < (uint32_t)P - 1) {
       const U hi = (U)((Ints<P>::NB >> 2) >> rp);
       const U lo = (U)((Ints<P>::NB >> 1) >> rp);
+      ZX_UNROLL
       for (int i = 0; i < 256; i++) u[i] += (uint32_t)i < rm ? hi : lo;
     }
   }

@@ -161,7 +161,7 @@ __device__ __forceinline__ void store_block(const FieldDesc &f, uint64_t b, bool
 
 // Encode every block; COUNT writes per-block bit lengths instead of bits.
 template <class S, int D, bool REV, bool COUNT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideEncodeWaves<S, REV>, 8))) void k_encode(EncodeArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4 ? 1 : kWideEncodeWaves<S, REV>, 8))) void k_encode(EncodeArgs a)
 {
   expand8_fill();
   const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideE
 }
 
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_decode(DecodeArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4 ? 1 : kWideDecodeWaves<S, REV>, 8))) void k_decode(DecodeArgs a)
 {
   compress8_fill<D>();
   const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
