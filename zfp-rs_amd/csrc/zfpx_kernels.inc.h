@@ -159,6 +159,9 @@ __device__ __forceinline__ void store_block(const FieldDesc &f, uint64_t b, bool
 
 // ------------------------------------------------------------------ global path
 
+// 4-D blocks run one wave per SIMD: a 32-bit block's 256 values take 256 of
+// the 512 registers (vector and accumulation registers) a lone wave may hold.
+
 // Encode every block; COUNT writes per-block bit lengths instead of bits.
 template <class S, int D, bool REV, bool COUNT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4 ? 1 : kWideEncodeWaves<S, REV>, 8))) void k_encode(EncodeArgs a)
