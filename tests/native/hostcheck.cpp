@@ -329,7 +329,8 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
                          uint64_t st, uint32_t fx, const uint32_t *l, uint32_t T)
 {
   FxPlan p;
-  if (dims == 4) {  // global path only
+  if (dims == 4) {  // fixed rate through LDS, else the global path
+    if (T && fx_plan(fx, st, true, T, p)) return decompress_fx<S, 4, REV>(f, c, w, nw, st, fx, p);
     decompress_t<S, 4, REV>(f, c, w, nw, st, fx, l);
     return 0;
   }

@@ -128,3 +128,9 @@ def test_4d_coder_many_blocks_and_budgets(ztype):
             got, _ = H.decompress(shape, arr.dtype, ztype, cfg, words, 0, fixed,
                                   None if fixed else lens, rev)
             assert np.array_equal(got.view(u), ref.view(u)), ctx
+            if fixed:
+                # the fixed-rate decode through LDS (k_decode_fx), whole-word
+                # and padded layouts
+                for T in (64, 7):
+                    got, _ = H.decompress(shape, arr.dtype, ztype, cfg, words, 0, fixed, None, rev, lds_T=T)
+                    assert np.array_equal(got.view(u), ref.view(u)), f"{ctx} T={T}"

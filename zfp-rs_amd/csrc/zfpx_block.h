@@ -926,9 +926,10 @@ ZX_HD uint64_t compress_ones(uint64_t v, uint32_t nbits)
 }
 
 // Dimensionalities whose decoders parse planes through the compress8 table
-// (64-bit windows and 32-bit windows for 3-D blocks, 32-bit windows for 2-D
-// blocks); a kernel that decodes them fills the workgroup's copy first.
-template <int D> constexpr bool kDecodeTable = D == 2 || D == 3;
+// (64-bit and 32-bit windows for 3-D blocks, 32-bit windows for 2-D blocks,
+// 64-bit windows for 4-D blocks); a kernel that decodes them fills the
+// workgroup's copy first.
+template <int D> constexpr bool kDecodeTable = D >= 2;
 
 // compress_ones of a value below 2^32: the four byte lookups only.
 ZX_HD uint32_t compress_ones32(uint32_t v)
@@ -1067,8 +1068,9 @@ ZX_HD uint32_t plane_code(uint64_t x, uint32_t n, uint64_t &clo, uint64_t &chi, 
 }
 
 // 4-D blocks: planes of 256 coefficients (bitplane.rs:161-218 keeps such a
-// plane in four words), coded as C's encode_ints and decode_ints are, one
-// group test at a time.  The planes come from the 64-coefficient chunk
+// plane in four words), coded as C's encode_ints is, one group test at a
+// time, and decoded a 64-bit window of tests at a time (plane_tests256).
+// The planes come from the 64-coefficient chunk
 // transposes applied to each quarter of the block, and decoded planes go back
 // through the 64-coefficient chunk folds, so every coefficient access has a
 // fixed index and the block stays in registers.
@@ -1178,6 +1180,79 @@ ZX_HD uint32_t encode_ints256(Sink &w, uint32_t maxbits, uint32_t maxprec,
   return budget - bits;
 }
 
+// v (new one-bits from coefficient n on, fewer than 64) ORed into a
+// 256-bit plane
+ZX_HD void or_at256(uint64_t (&x)[4], uint64_t v, uint32_t n)
+{
+  const uint32_t w = n >> 6, sh = n & 63;
+  const uint64_t lo = v << sh;
+  const uint64_t hi = (v >> 1) >> (63 - sh);  // v >> (64 - sh), 0 when sh == 0
+  ZX_UNROLL
+  for (int q = 0; q < 4; q++) x[q] |= (w == (uint32_t)q ? lo : 0ull) | (w + 1 == (uint32_t)q ? hi : 0ull);
+}
+
+// The group tests of a 256-coefficient plane: plane_tests' 64-bit window
+// parse (below), each lane on its own, with the plane held in four words.
+// The reader sits at the plane's first test, n < 256 and bits > 0.
+template <class Src>
+ZX_HD void plane_tests256(Src &r, uint32_t &n, uint32_t &bits, uint64_t (&x)[4], bool &one)
+{
+  constexpr uint64_t ODD = 0xaaaaaaaaaaaaaaaaull;
+  one = false;
+  for (uint32_t s = 1;; s = 0) {
+    const uint64_t w = r.peek();
+    if (s && !(w & 1u)) {  // negative first test
+      r.consume(1);
+      bits -= 1;
+      return;
+    }
+    const uint64_t y = w >> s;
+    const uint32_t W = 64 - s;
+    const uint64_t z = (y + (y & ~(y << 1) & ODD)) ^ ODD;
+    const uint64_t val = y & z;
+    const uint64_t term = ~y & (y << 1) & z & lowmask(W);
+    const bool done = term != 0;
+    const uint32_t e = ctz64(term);
+    const uint32_t c = done ? e + 1 : W - (uint32_t)((val >> (W - 1)) & 1u);
+    const uint32_t vr = done ? e : c;
+    const uint64_t vals = val & lowmask(vr);
+    const uint64_t xn = compress_ones(vals, vr);
+    const uint32_t coef = done ? c - popc64(vals) : c - popc64(val & lowmask(c - 1));
+    // a zero scan does not read coefficient 255: it is implied
+    const uint32_t q = 255 - n;
+    const bool size_hit = n + coef >= 256;  // then q < coef <= 64
+    const uint32_t len = size_hit ? q + popc64(xn & lowmask(q)) : c;
+    const uint32_t yb = bits - s;
+    if (len > yb) {
+      // the budget ends after yb bits: a one is implied unless the last bit
+      // read was a value
+      const uint32_t nt = yb ? yb - popc64(val & lowmask(yb - 1)) : 0u;
+      const bool lastv = yb != 0 && ((val >> (yb - 1)) & 1u);
+      or_at256(x, xn & lowmask(nt), n);
+      n += nt;
+      if (!lastv) or_at256(x, 1ull, n++);
+      r.consume(s + yb);
+      bits = 0;
+      one = true;
+      return;
+    }
+    if (size_hit) {
+      or_at256(x, xn & lowmask(q), n);
+      x[3] |= 1ull << 63;
+      n = 256;
+      r.consume(s + len);
+      bits -= s + len;
+      one = true;
+      return;
+    }
+    or_at256(x, xn, n);
+    n += coef;
+    r.consume(s + c);
+    bits -= s + c;
+    if (done) return;
+  }
+}
+
 template <int P, class Src>
 ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool round_last,
                               typename Ints<P>::U (&u)[256], bool &skip)
@@ -1213,30 +1288,7 @@ ZX_HD uint32_t decode_ints256(Src &r, uint32_t maxbits, uint32_t maxprec, bool r
           if (m > lo) x[q] = r.read(umin(m - lo, 64u));
         }
         tested = bits != 0 && n < 256;
-        while (bits != 0 && n < 256) {
-          bits -= 1;
-          one = r.read(1) != 0;
-          if (!one) break;
-          // the zeros before the next one-bit (read up to 64 at a time), up
-          // to the last coefficient, whose one-bit is implied, or the budget
-          for (uint32_t left = umin(255 - n, bits); left != 0;) {
-            const uint32_t span = umin(left, 64);
-            const uint32_t z = ctz64(r.peek());
-            if (z < span) {
-              r.consume(z + 1);
-              bits -= z + 1;
-              n += z;
-              break;
-            }
-            r.consume(span);
-            bits -= span;
-            n += span;
-            left -= span;
-          }
-          ZX_UNROLL
-          for (int q = 0; q < 4; q++) x[q] |= (n >> 6) == (uint32_t)q ? 1ull << (n & 63) : 0ull;
-          n++;
-        }
+        if (tested) plane_tests256(r, n, bits, x, one);
       }
       if (tested) m = one ? n : 256u;
       ZX_UNROLL

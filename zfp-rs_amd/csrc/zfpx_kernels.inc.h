@@ -40,6 +40,9 @@ template <class S, bool REV> constexpr int kMinWaves = ST<S>::P == 32 && !REV ? 
 template <class S, bool REV> constexpr int kWideDecodeWaves = ST<S>::P == 64 && !REV ? 2 : 1;
 template <class S, bool REV>
 constexpr int kDecodeWaves = ST<S>::P == 32 ? kMinWaves<S, REV> : kWideDecodeWaves<S, REV>;
+// 4-D blocks run one wave per SIMD: a 32-bit block's 256 values take 256 of
+// the 512 registers (vector and accumulation registers) a lone wave may hold.
+template <class S, int D, bool REV> constexpr int kFxDecodeWaves = D == 4 ? 1 : kDecodeWaves<S, REV>;
 
 // Waves per SIMD the encoders are held to: 32-bit as kMinWaves; 64-bit lossy
 // two (256 registers, a few values spilled), where unconstrained the
@@ -159,9 +162,6 @@ __device__ __forceinline__ void store_block(const FieldDesc &f, uint64_t b, bool
 
 // ------------------------------------------------------------------ global path
 
-// 4-D blocks run one wave per SIMD: a 32-bit block's 256 values take 256 of
-// the 512 registers (vector and accumulation registers) a lone wave may hold.
-
 // Encode every block; COUNT writes per-block bit lengths instead of bits.
 template <class S, int D, bool REV, bool COUNT>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4 ? 1 : kWideEncodeWaves<S, REV>, 8))) void k_encode(EncodeArgs a)
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kEncod
 // once per block): the group-test loop waits on LDS reads, and the extra wave
 // hides more of that than the spills cost.
 template <class S, int D, bool REV, bool FLAT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecodeWaves<S, REV>, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kFxDecodeWaves<S, D, REV>, 8))) void k_decode_fx(DecodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
@@ -577,14 +577,14 @@ template <class S, int D, bool REV>
 static hipError_t launch_decode_t(const DecodeArgs &a, hipStream_t s)
 {
   if (a.field.nblocks == 0) return hipSuccess;
+  FxPlan p;
+  if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p)) {
+    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+    if (p.flat) hipLaunchKernelGGL((k_decode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+    else hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+    return hipGetLastError();
+  }
   if constexpr (D != 4) {
-    FxPlan p;
-    if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p)) {
-      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
-      if (p.flat) hipLaunchKernelGGL((k_decode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-      else hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-      return hipGetLastError();
-    }
     if (a.offsets && a.vbits && !a.count) {
       // one wave per workgroup, as the fixed-rate decode
       vx_plan(a.vbits, a.field.nblocks, a.slice ? a.slice : 64u, 0, p);
