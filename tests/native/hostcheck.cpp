@@ -302,7 +302,10 @@ static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, 
                        uint64_t st, uint32_t fx, uint32_t *l, uint32_t T, uint32_t cap)
 {
   FxPlan p;
-  if (dims == 4) return compress_t<S, 4, REV>(f, c, w, nw, st, fx, l);  // global path only
+  if (dims == 4) {  // fixed rate through LDS, else the global path
+    if (T && fx_plan(fx, st, false, T, p)) return compress_fx<S, 4, REV>(f, c, w, nw, st, fx, p);
+    return compress_t<S, 4, REV>(f, c, w, nw, st, fx, l);
+  }
   if (T && !fx) {
     switch (dims) {
     case 1: return compress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);

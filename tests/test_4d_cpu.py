@@ -129,8 +129,11 @@ def test_4d_coder_many_blocks_and_budgets(ztype):
                                   None if fixed else lens, rev)
             assert np.array_equal(got.view(u), ref.view(u)), ctx
             if fixed:
-                # the fixed-rate decode through LDS (k_decode_fx), whole-word
-                # and padded layouts
+                # the fixed-rate compress and decode through LDS (k_encode_fx,
+                # k_decode_fx), whole-word and padded layouts
                 for T in (64, 7):
+                    w2 = np.zeros(nwords + 4, dtype=np.uint64)
+                    H.compress(arr, ztype, cfg, w2, 0, fixed, rev, lds_T=T)
+                    assert np.array_equal(w2, words), f"{ctx} T={T}"
                     got, _ = H.decompress(shape, arr.dtype, ztype, cfg, words, 0, fixed, None, rev, lds_T=T)
                     assert np.array_equal(got.view(u), ref.view(u)), f"{ctx} T={T}"

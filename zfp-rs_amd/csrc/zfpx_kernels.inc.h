@@ -51,6 +51,7 @@ template <class S, int D, bool REV> constexpr int kFxDecodeWaves = D == 4 ? 1 : 
 template <class S, bool REV> constexpr int kWideEncodeWaves = ST<S>::P == 64 && !REV ? 2 : 1;
 template <class S, bool REV>
 constexpr int kEncodeWaves = ST<S>::P == 32 ? kMinWaves<S, REV> : kWideEncodeWaves<S, REV>;
+template <class S, int D, bool REV> constexpr int kFxEncodeWaves = D == 4 ? 1 : kEncodeWaves<S, REV>;
 
 // Fills the workgroup's expand8 table (zfpx_block.h); every kernel that
 // encodes calls it first, with all of its threads.
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4
 // blockDim.x blocks.  32-bit values fit four waves per SIMD without spilling
 // (the coder keeps packed chunk halves rather than the coefficients).
 template <class S, int D, bool REV, bool OWN>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kEncodeWaves<S, REV>, 8))) void k_encode_fx(EncodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kFxEncodeWaves<S, D, REV>, 8))) void k_encode_fx(EncodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   expand8_fill();
@@ -544,6 +545,14 @@ template <class S, int D, bool REV>
 static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
 {
   if (a.field.nblocks == 0) return hipSuccess;
+  FxPlan p;
+  if (!a.op_state && !a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p)) {
+    const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+    const size_t bytes = (size_t)p.lds_words * 8;
+    if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
+    else hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
+    return hipGetLastError();
+  }
   if constexpr (D == 4) {
     if (a.op_state) return hipErrorNotSupported;
     return launch_global_encode_t<S, D, REV>(a, s);
@@ -552,14 +561,6 @@ static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
       const OnePlan p = op_plan(block_bound(a.cfg, ST<S>::P, Blk<D>::SIZE), 0, a.op_T, a.vcap);
       const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
       hipLaunchKernelGGL((k_encode_1p<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-      return hipGetLastError();
-    }
-    FxPlan p;
-    if (!a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p)) {
-      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
-      const size_t bytes = (size_t)p.lds_words * 8;
-      if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
-      else hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
       return hipGetLastError();
     }
     if (!a.lengths && a.offsets && a.vbits) {
