@@ -302,28 +302,27 @@ static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, 
                        uint64_t st, uint32_t fx, uint32_t *l, uint32_t T, uint32_t cap)
 {
   FxPlan p;
-  if (dims == 4) {  // fixed rate through LDS, else the global path
-    if (T && fx_plan(fx, st, false, T, p)) return compress_fx<S, 4, REV>(f, c, w, nw, st, fx, p);
-    return compress_t<S, 4, REV>(f, c, w, nw, st, fx, l);
-  }
   if (T && !fx) {
     switch (dims) {
     case 1: return compress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);
     case 2: return compress_vx<S, 2, REV>(f, c, w, nw, st, l, T, cap);
-    default: return compress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
+    case 3: return compress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
+    default: return compress_vx<S, 4, REV>(f, c, w, nw, st, l, T, cap);
     }
   }
   if (T && fx_plan(fx, st, false, T, p)) {
     switch (dims) {
     case 1: return compress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
     case 2: return compress_fx<S, 2, REV>(f, c, w, nw, st, fx, p);
-    default: return compress_fx<S, 3, REV>(f, c, w, nw, st, fx, p);
+    case 3: return compress_fx<S, 3, REV>(f, c, w, nw, st, fx, p);
+    default: return compress_fx<S, 4, REV>(f, c, w, nw, st, fx, p);
     }
   }
   switch (dims) {
   case 1: return compress_t<S, 1, REV>(f, c, w, nw, st, fx, l);
   case 2: return compress_t<S, 2, REV>(f, c, w, nw, st, fx, l);
-  default: return compress_t<S, 3, REV>(f, c, w, nw, st, fx, l);
+  case 3: return compress_t<S, 3, REV>(f, c, w, nw, st, fx, l);
+  default: return compress_t<S, 4, REV>(f, c, w, nw, st, fx, l);
   }
 }
 
@@ -332,16 +331,12 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
                          uint64_t st, uint32_t fx, const uint32_t *l, uint32_t T)
 {
   FxPlan p;
-  if (dims == 4) {  // fixed rate through LDS, else the global path
-    if (T && fx_plan(fx, st, true, T, p)) return decompress_fx<S, 4, REV>(f, c, w, nw, st, fx, p);
-    decompress_t<S, 4, REV>(f, c, w, nw, st, fx, l);
-    return 0;
-  }
   if (T && fx_plan(fx, st, true, T, p)) {
     switch (dims) {
     case 1: return decompress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
     case 2: return decompress_fx<S, 2, REV>(f, c, w, nw, st, fx, p);
-    default: return decompress_fx<S, 3, REV>(f, c, w, nw, st, fx, p);
+    case 3: return decompress_fx<S, 3, REV>(f, c, w, nw, st, fx, p);
+    default: return decompress_fx<S, 4, REV>(f, c, w, nw, st, fx, p);
     }
   }
   if (T && !fx && l) {
@@ -353,13 +348,15 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
     switch (dims) {
     case 1: return decompress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);
     case 2: return decompress_vx<S, 2, REV>(f, c, w, nw, st, l, T, cap);
-    default: return decompress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
+    case 3: return decompress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
+    default: return decompress_vx<S, 4, REV>(f, c, w, nw, st, l, T, cap);
     }
   }
   switch (dims) {
   case 1: decompress_t<S, 1, REV>(f, c, w, nw, st, fx, l); break;
   case 2: decompress_t<S, 2, REV>(f, c, w, nw, st, fx, l); break;
-  default: decompress_t<S, 3, REV>(f, c, w, nw, st, fx, l); break;
+  case 3: decompress_t<S, 3, REV>(f, c, w, nw, st, fx, l); break;
+  default: decompress_t<S, 4, REV>(f, c, w, nw, st, fx, l); break;
   }
   return 0;
 }
