@@ -406,7 +406,7 @@ def four_d_leg(steps=3):
     """4-D fields: 64^4 f32 (a smooth separable field plus 1e-3 noise) at fixed
     rate 8 and fixed precision 16, and 32^4 f64 reversible; compress and
     decompress GB/s as for the variable-rate configurations.  4-D blocks run
-    the global-memory kernels with their 256 values in scratch memory."""
+    one wave per SIMD, a 32-bit block's 256 values in registers."""
     import torch
 
     import zfp_hip as Z

@@ -530,8 +530,10 @@ __global__ __launch_bounds__(64) void k_batch_walk(BatchArgs a)
 
 // ------------------------------------------------------------------ launchers
 
-// 4-D blocks (256 values) keep to the global-memory kernels and have no batch
-// launches.
+// The global-memory kernels: every block coded from and to HBM at its own
+// position (variable-rate length and write passes, decodes from offsets,
+// blocks too large to stage).  4-D blocks (256 values) have no batch
+// launches and no one-pass compress.
 template <class S, int D, bool REV>
 static hipError_t launch_global_encode_t(const EncodeArgs &a, hipStream_t s)
 {
