@@ -302,15 +302,15 @@ static uint64_t comp_d(int dims, const FieldDesc &f, const Cfg &c, uint64_t *w, 
                        uint64_t st, uint32_t fx, uint32_t *l, uint32_t T, uint32_t cap)
 {
   FxPlan p;
-  if (T && !fx) {
+  if (T && !fx && dims < 4) {
     switch (dims) {
     case 1: return compress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);
     case 2: return compress_vx<S, 2, REV>(f, c, w, nw, st, l, T, cap);
-    case 3: return compress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
-    default: return compress_vx<S, 4, REV>(f, c, w, nw, st, l, T, cap);
+    default: return compress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
     }
   }
-  if (T && fx_plan(fx, st, false, T, p)) {
+  // (4-D: the whole-block layout only, as the kernels' launcher)
+  if (T && fx_plan(fx, st, false, T, p) && (dims < 4 || p.own)) {
     switch (dims) {
     case 1: return compress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
     case 2: return compress_fx<S, 2, REV>(f, c, w, nw, st, fx, p);
@@ -331,7 +331,7 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
                          uint64_t st, uint32_t fx, const uint32_t *l, uint32_t T)
 {
   FxPlan p;
-  if (T && fx_plan(fx, st, true, T, p)) {
+  if (T && fx_plan(fx, st, true, T, p) && (dims < 4 || p.flat)) {
     switch (dims) {
     case 1: return decompress_fx<S, 1, REV>(f, c, w, nw, st, fx, p);
     case 2: return decompress_fx<S, 2, REV>(f, c, w, nw, st, fx, p);
@@ -339,7 +339,7 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
     default: return decompress_fx<S, 4, REV>(f, c, w, nw, st, fx, p);
     }
   }
-  if (T && !fx && l) {
+  if (T && !fx && l && dims < 4) {
     // odd T: a capacity of the mean group's words, so that some groups are
     // staged and some left to the dense pass
     uint64_t bits = 0;
@@ -348,8 +348,7 @@ static uint64_t decomp_d(int dims, const FieldDesc &f, const Cfg &c, const uint6
     switch (dims) {
     case 1: return decompress_vx<S, 1, REV>(f, c, w, nw, st, l, T, cap);
     case 2: return decompress_vx<S, 2, REV>(f, c, w, nw, st, l, T, cap);
-    case 3: return decompress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
-    default: return decompress_vx<S, 4, REV>(f, c, w, nw, st, l, T, cap);
+    default: return decompress_vx<S, 3, REV>(f, c, w, nw, st, l, T, cap);
     }
   }
   switch (dims) {

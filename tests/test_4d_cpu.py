@@ -128,21 +128,10 @@ def test_4d_coder_many_blocks_and_budgets(ztype):
             got, _ = H.decompress(shape, arr.dtype, ztype, cfg, words, 0, fixed,
                                   None if fixed else lens, rev)
             assert np.array_equal(got.view(u), ref.view(u)), ctx
-            if not fixed:
-                # the variable-rate write pass and the decode through the
-                # offsets staged by workgroup (k_encode_vx / k_decode_vx, the
-                # groups over the capacity by the dense kernels): odd T sets a
-                # capacity of the mean group, so both kinds of group run
-                for T in (64, 5):
-                    cap = (int(offs[-1]) * T // 24 + 63) // 64 + 1 if T % 2 else 0
-                    w2 = np.zeros(nwords + 4, dtype=np.uint64)
-                    H.compress(arr, ztype, cfg, w2, 0, 0, rev, lds_T=T, lds_cap=cap)
-                    assert np.array_equal(w2, words), f"{ctx} T={T}"
-                    got, _ = H.decompress(shape, arr.dtype, ztype, cfg, words, 0, 0, lens, rev, lds_T=T)
-                    assert np.array_equal(got.view(u), ref.view(u)), f"{ctx} T={T}"
             if fixed:
                 # the fixed-rate compress and decode through LDS (k_encode_fx,
-                # k_decode_fx), whole-word and padded layouts
+                # k_decode_fx; 4-D blocks take the whole-block layouts only,
+                # and fall back to the global kernels otherwise)
                 for T in (64, 7):
                     w2 = np.zeros(nwords + 4, dtype=np.uint64)
                     H.compress(arr, ztype, cfg, w2, 0, fixed, rev, lds_T=T)

@@ -148,15 +148,6 @@ ZX_HD void vx_plan(uint64_t bits, uint64_t nblocks, uint32_t T, uint32_t cap, Fx
   p.lds_words = fx_slot(p.cap + 2, p.skew) + 1;
 }
 
-// Whether a group of the mean length fits the plan's capacity: where LDS
-// holds less (large 4-D blocks), most groups would be left to the dense
-// kernels, which take a workgroup's groups one after another.
-ZX_HD bool vx_fits(uint64_t bits, uint64_t nblocks, const FxPlan &p)
-{
-  const uint64_t mean = nblocks ? (bits + nblocks - 1) / nblocks : 0;
-  return (mean * p.T + 63) / 64 + 1 <= p.cap;
-}
-
 ZX_HD FxGroup vx_group(uint64_t g, uint32_t T, uint64_t nblocks, const uint64_t *offsets)
 {
   FxGroup q;

@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kFxDec
 // p.cap words (by the offsets) stages them, its loads batched, and decodes
 // from LDS; the groups over the capacity are left to k_decode_dense.
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kFxDecodeWaves<S, D, REV>, 8))) void k_decode_vx(DecodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kDecodeWaves<S, REV>, 8))) void k_decode_vx(DecodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kFxDec
 // as k_decode does; a workgroup tests blockDim.x groups at once (one per
 // thread) and then decodes the dense ones among them.
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4 ? 1 : kWideDecodeWaves<S, REV>, 8))) void k_decode_dense(DecodeArgs a, FxPlan p, uint64_t ngroups)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideDecodeWaves<S, REV>, 8))) void k_decode_dense(DecodeArgs a, FxPlan p, uint64_t ngroups)
 {
   __shared__ uint32_t dense[kThreads];
   __shared__ uint32_t ndense;
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4
 // its neighbours).  Denser workgroups are left to k_encode_dense.  The choice
 // is uniform over the workgroup, so the barriers are too.
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kFxEncodeWaves<S, D, REV>, 8))) void k_encode_vx(EncodeArgs a, FxPlan p)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kEncodeWaves<S, REV>, 8))) void k_encode_vx(EncodeArgs a, FxPlan p)
 {
   extern __shared__ uint64_t lds[];
   const uint32_t t = threadIdx.x;
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kFxEnc
 // workgroup first tests blockDim.x groups at once (one per thread) and then
 // codes the dense ones among them.
 template <class S, int D, bool REV>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(D == 4 ? 1 : kWideEncodeWaves<S, REV>, 8))) void k_encode_dense(EncodeArgs a, FxPlan p, uint64_t ngroups)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kWideEncodeWaves<S, REV>, 8))) void k_encode_dense(EncodeArgs a, FxPlan p, uint64_t ngroups)
 {
   __shared__ uint32_t dense[kThreads];
   __shared__ uint32_t ndense;
@@ -547,27 +547,29 @@ template <class S, int D, bool REV>
 static hipError_t launch_encode_t(const EncodeArgs &a, hipStream_t s)
 {
   if (a.field.nblocks == 0) return hipSuccess;
+  // 4-D blocks take the staged fixed-rate encode in its whole-block layout
+  // only and the global-memory kernels otherwise: every 4-D kernel holds a
+  // 256-value codec, and each one instantiated adds minutes to the build
   FxPlan p;
-  if (!a.op_state && !a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p)) {
+  if (!a.op_state && !a.lengths && !a.offsets && fx_plan(a.fixed, a.start, false, a.slice, p) &&
+      (D != 4 || p.own)) {
     const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
     const size_t bytes = (size_t)p.lds_words * 8;
     if (p.own) hipLaunchKernelGGL((k_encode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), bytes, s, a, p);
-    else hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
+    else if constexpr (D != 4) hipLaunchKernelGGL((k_encode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), bytes, s, a, p);
     return hipGetLastError();
-  }
-  if (!a.op_state && !a.lengths && a.offsets && a.vbits) {
-    vx_plan(a.vbits, a.field.nblocks, a.slice, a.vcap, p);
-    if (D != 4 || vx_fits(a.vbits, a.field.nblocks, p)) {
-      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
-      hipLaunchKernelGGL((k_encode_vx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-      hipLaunchKernelGGL((k_encode_dense<S, D, REV>), dim3((grid + p.T - 1) / p.T), dim3(p.T), 0, s, a, p, grid);
-      return hipGetLastError();
-    }
   }
   if constexpr (D == 4) {
     // (4-D fields keep the length pass, scan and write pass)
     if (a.op_state) return hipErrorNotSupported;
   } else {
+    if (!a.op_state && !a.lengths && a.offsets && a.vbits) {
+      vx_plan(a.vbits, a.field.nblocks, a.slice, a.vcap, p);
+      const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
+      hipLaunchKernelGGL((k_encode_vx<S, D, REV>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+      hipLaunchKernelGGL((k_encode_dense<S, D, REV>), dim3((grid + p.T - 1) / p.T), dim3(p.T), 0, s, a, p, grid);
+      return hipGetLastError();
+    }
     if (a.op_state) {
       const OnePlan p = op_plan(block_bound(a.cfg, ST<S>::P, Blk<D>::SIZE), 0, a.op_T, a.vcap);
       const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
@@ -582,17 +584,19 @@ template <class S, int D, bool REV>
 static hipError_t launch_decode_t(const DecodeArgs &a, hipStream_t s)
 {
   if (a.field.nblocks == 0) return hipSuccess;
+  // 4-D blocks: the staged fixed-rate decode of whole-word blocks only (as
+  // the encode above)
   FxPlan p;
-  if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p)) {
+  if (!a.offsets && fx_plan(a.fixed, a.start, true, a.slice, p) && (D != 4 || p.flat)) {
     const uint64_t grid = (a.field.nblocks + p.T - 1) / p.T;
     if (p.flat) hipLaunchKernelGGL((k_decode_fx<S, D, REV, true>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
-    else hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
+    else if constexpr (D != 4) hipLaunchKernelGGL((k_decode_fx<S, D, REV, false>), dim3(grid), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
     return hipGetLastError();
   }
-  if (a.offsets && a.vbits && !a.count) {
-    // one wave per workgroup, as the fixed-rate decode
-    vx_plan(a.vbits, a.field.nblocks, a.slice ? a.slice : 64u, 0, p);
-    if (D != 4 || vx_fits(a.vbits, a.field.nblocks, p)) {
+  if constexpr (D != 4) {
+    if (a.offsets && a.vbits && !a.count) {
+      // one wave per workgroup, as the fixed-rate decode
+      vx_plan(a.vbits, a.field.nblocks, a.slice ? a.slice : 64u, 0, p);
       const uint64_t groups = (a.field.nblocks + p.T - 1) / p.T;
       hipLaunchKernelGGL((k_decode_vx<S, D, REV>), dim3(groups), dim3(p.T), (size_t)p.lds_words * 8, s, a, p);
       hipLaunchKernelGGL((k_decode_dense<S, D, REV>), dim3((groups + kThreads - 1) / kThreads), dim3(kThreads), 0,
