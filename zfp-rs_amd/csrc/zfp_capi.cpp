@@ -1329,6 +1329,12 @@ bool block_plan(zfp_type t, int dims, const size_t len[4], const ptrdiff_t str[4
     else p.imax += (int64_t)ext;
   }
   for (int a = 0; a < 3; a++) p.nb[a] = 1;
+  p.nelems = (uint64_t)(p.n[0] * p.n[1] * p.n[2] * p.n[3]);
+  p.fits = true;
+  // one thread writes the block's values in the reference's order, so
+  // strides that repeat an element leave its last value without the dense
+  // copy and scatter of an aliasing field
+  p.may_alias = false;
   return true;
 }
 
@@ -1342,7 +1348,7 @@ size_t block_code(bool encode, zfp_stream *z, const void *block, zfp_type t, int
                   const size_t len[4], const ptrdiff_t str[4])
 {
   if (!z || !z->stream || !block) return 0;
-  Plan p;
+  Plan p{};
   if (!block_plan(t, dims, len, str, p)) return 0;
   if (((uintptr_t)((const char *)block + p.imin * (int64_t)p.esz)) % p.esz) return 0;
   zfp_field f{};
